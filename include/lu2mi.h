@@ -1,0 +1,101 @@
+/*
+ * lu2mi.h -- C ABI of the MI355X-native lu2 LU library (liblu2mi.so).
+ *
+ * The library factors lu2's augmented matrix on one GPU: row-major,
+ * leading dimension lda >= n+1, right-hand side b stored in column n
+ * (lu2.c:28 RDIM, lu2.c:70-85).  The stored factor form is lu2's
+ * (lu2.c:524-581, lu2.c:1088-1120, lu2tlib.c:1041-1128): per nb-block the
+ * pivot rows are scaled by 1/d (unit-upper U), the diagonal block holds
+ * D^-1 L11 below the diagonal and d on it, rows below keep L21 unscaled,
+ * and earlier L columns are never re-swapped.  Column n ends up as the
+ * right-hand side of U x = y and x comes from the unit-upper back
+ * substitution (lu2.c:110-116).
+ *
+ * All functions return LU2MI_OK (0) or a negative error code;
+ * lu2mi_last_error() gives a message.  Pointers named d* are device
+ * pointers (from lu2mi_malloc or any hipMalloc on the context's device).
+ * Calls are synchronous unless stated otherwise.  A context may be used by
+ * one host thread at a time.
+ */
+#ifndef LU2MI_H
+#define LU2MI_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LU2MI_OK               0
+#define LU2MI_ERR_ARG         -1
+#define LU2MI_ERR_HIP         -2
+#define LU2MI_ERR_NOMEM       -3
+#define LU2MI_ERR_DEVICE      -4   /* device-side failure (barrier time-out) */
+#define LU2MI_ERR_UNSUPPORTED -5
+#define LU2MI_ERR_KAT         -6   /* known-answer test failed */
+#define LU2MI_ERR_NOCONV      -7   /* iterative refinement did not converge */
+
+/* Pivot rules.  COMPAT reproduces lu2's leaf search, which reads column 0
+ * of each 2-column leaf for both columns (lu2tlib.c:42-52 via
+ * lu2tlib.c:1050), so odd column j pivots on argmax_{r>=j}|a[r][j-1]|.
+ * GEPP is standard partial pivoting.  Ties go to the lowest row. */
+#define LU2MI_PIV_COMPAT 0
+#define LU2MI_PIV_GEPP   1
+
+/* options for lu2mi_set_option */
+#define LU2MI_OPT_PIVMODE  1   /* LU2MI_PIV_*; default COMPAT (the reference's rule) */
+#define LU2MI_OPT_LEAF_W   2   /* panel leaf width: 16 or 32 (default 32) */
+#define LU2MI_OPT_TIMING   3   /* 1: record HIP events around trailing updates */
+
+typedef struct lu2mi_ctx lu2mi_ctx;
+
+int         lu2mi_init(int device, lu2mi_ctx **ctx);
+int         lu2mi_finalize(lu2mi_ctx *ctx);
+const char *lu2mi_last_error(void);
+int         lu2mi_set_option(lu2mi_ctx *ctx, int opt, int value);
+int         lu2mi_synchronize(lu2mi_ctx *ctx);
+int         lu2mi_device_info(lu2mi_ctx *ctx, int *num_cu, char *arch, int arch_len);
+
+/* device memory helpers */
+int lu2mi_malloc(lu2mi_ctx *ctx, void **dptr, size_t bytes);
+int lu2mi_free(lu2mi_ctx *ctx, void *dptr);
+int lu2mi_memcpy_h2d(lu2mi_ctx *ctx, void *ddst, const void *hsrc, size_t bytes);
+int lu2mi_memcpy_d2h(lu2mi_ctx *ctx, void *hdst, const void *dsrc, size_t bytes);
+int lu2mi_memcpy_d2d(lu2mi_ctx *ctx, void *ddst, const void *dsrc, size_t bytes);
+
+/* lu2.c:70-85 randomsetmat on the device: a[i][j] = drand48()-0.5 in row
+ * order with n+1 draws per row (b last), bit-identical to glibc drand48
+ * after srand48(seed).  Columns n+1..lda-1 are set to 0. */
+int lu2mi_randomsetmat(lu2mi_ctx *ctx, int n, int seed, double *dA, long lda);
+
+/* lu2.c:1122-1144 lumcolumn(n, a, b, nb, awork, pv, 1), GPU-resident:
+ * factors dA in place (lu2's stored form, pivot rule from
+ * LU2MI_OPT_PIVMODE) and writes x to dx (may be NULL).  dpiv (may be NULL)
+ * receives the full global pivot sequence, dpiv[j] = row exchanged with
+ * row j at step j; the reference keeps only the last block's (lu2.c:1128).
+ * Requires nb a power of two, 32 <= nb, n % nb == 0 (reference INSTALL:54-58). */
+int lu2mi_lumcolumn_dev(lu2mi_ctx *ctx, int n, double *dA, long lda, int nb,
+                        int *dpiv, double *dx);
+
+/* lu2lib.c:2079-2135 mydgemm on device pointers: row-major
+ * C[m x n] = alpha*A[m x k]*B[k x n] + beta*C, FP64 MFMA. */
+int lu2mi_dgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, double alpha,
+                    const double *dA, long lda, const double *dB, long ldb,
+                    double beta, double *dC, long ldc);
+
+/* FP64 residual on the device against the unfactored augmented matrix
+ * dA0 (b in column n).  out[0]=||Ax-b||_oo out[1]=||A||_oo out[2]=||A||_1
+ * out[3]=||x||_oo out[4]=||x||_1 out[5]=||b||_oo
+ * out[6]=HPL-2 scaled residual ||Ax-b||_oo/(eps(||A||_oo||x||_oo+||b||_oo)n),
+ * eps=2^-53; out[7..9] = lu2_mpi.c:2170-2176 resid1..3 (eps=2e-16). */
+int lu2mi_residual_dev(lu2mi_ctx *ctx, int n, const double *dA0, long lda,
+                       const double *dx, double *out);
+
+/* Timing of the last lu2mi_lumcolumn_dev with LU2MI_OPT_TIMING on:
+ * out[0]=total ms of trailing-update GEMMs (HIP events on the library
+ * stream), out[1]=number of launches, out[2]=their algorithmic flops,
+ * out[3]=wall ms of the whole call. */
+int lu2mi_get_timing(lu2mi_ctx *ctx, double *out, int nout);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,190 @@
+"""lu2_amd -- MI355X-native lu2 LU + solve.
+
+Python bindings (ctypes) to liblu2mi.so, the C ABI declared in
+include/lu2mi.h and include/lu2mi_seams.h.  The compute runs in the HIP
+kernels of lu2_amd/csrc; this module only moves pointers and sizes.  If the
+library has not been built, every call raises: there is no CPU fallback.
+
+Mirrors the reference driver's operations (lu2.c):
+  randomsetmat (lu2.c:70-85)   -> Context.randomsetmat
+  lumcolumn    (lu2.c:1122)    -> Context.lumcolumn_dev / solve()
+  mydgemm      (lu2lib.c:2079) -> Context.dgemm_dev
+  showresult   (lu2.c:45-60)   -> Context.residual_dev
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblu2mi.so")
+
+PIV_COMPAT = 0
+PIV_GEPP = 1
+OPT_PIVMODE = 1
+OPT_LEAF_W = 2
+OPT_TIMING = 3
+
+_lib = None
+
+
+class Lu2miError(RuntimeError):
+    pass
+
+
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    """Load liblu2mi.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise Lu2miError(f"{p} not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(p)
+    c_int, c_long, c_double, c_size = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_size_t
+    vp = ctypes.c_void_p
+    sig = {
+        "lu2mi_init": (c_int, [c_int, ctypes.POINTER(vp)]),
+        "lu2mi_finalize": (c_int, [vp]),
+        "lu2mi_last_error": (ctypes.c_char_p, []),
+        "lu2mi_set_option": (c_int, [vp, c_int, c_int]),
+        "lu2mi_synchronize": (c_int, [vp]),
+        "lu2mi_device_info": (c_int, [vp, ctypes.POINTER(c_int), ctypes.c_char_p, c_int]),
+        "lu2mi_malloc": (c_int, [vp, ctypes.POINTER(vp), c_size]),
+        "lu2mi_free": (c_int, [vp, vp]),
+        "lu2mi_memcpy_h2d": (c_int, [vp, vp, vp, c_size]),
+        "lu2mi_memcpy_d2h": (c_int, [vp, vp, vp, c_size]),
+        "lu2mi_memcpy_d2d": (c_int, [vp, vp, vp, c_size]),
+        "lu2mi_randomsetmat": (c_int, [vp, c_int, c_int, vp, c_long]),
+        "lu2mi_lumcolumn_dev": (c_int, [vp, c_int, vp, c_long, c_int, vp, vp]),
+        "lu2mi_dgemm_dev": (c_int, [vp, c_int, c_int, c_int, c_double, vp, c_long, vp, c_long,
+                                    c_double, vp, c_long]),
+        "lu2mi_residual_dev": (c_int, [vp, c_int, vp, c_long, vp, vp]),
+        "lu2mi_get_timing": (c_int, [vp, vp, c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = _lib.lu2mi_last_error().decode(errors="replace") if _lib else ""
+        raise Lu2miError(f"liblu2mi error {rc}: {msg}")
+
+
+class DeviceBuffer:
+    """A device allocation owned by a Context."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(_lib.lu2mi_malloc(ctx.handle, ctypes.byref(p), ctypes.c_size_t(self.nbytes)))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        _check(_lib.lu2mi_memcpy_h2d(self.ctx.handle, self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(_lib.lu2mi_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int | None = None) -> None:
+        nb = other.nbytes if nbytes is None else nbytes
+        _check(_lib.lu2mi_memcpy_d2d(self.ctx.handle, self.ptr, other.ptr, nb))
+
+    def free(self) -> None:
+        if self.ptr:
+            _check(_lib.lu2mi_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """One GPU, one HIP stream, the factorization workspace."""
+
+    def __init__(self, device: int = 0):
+        load_library()
+        h = ctypes.c_void_p()
+        _check(_lib.lu2mi_init(device, ctypes.byref(h)))
+        self.handle = h.value
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.lu2mi_finalize(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_option(self, opt: int, value: int) -> None:
+        _check(_lib.lu2mi_set_option(self.handle, opt, value))
+
+    def device_info(self):
+        ncu = ctypes.c_int()
+        arch = ctypes.create_string_buffer(64)
+        _check(_lib.lu2mi_device_info(self.handle, ctypes.byref(ncu), arch, 64))
+        return ncu.value, arch.value.decode()
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def randomsetmat(self, n: int, seed: int, dA: DeviceBuffer, lda: int) -> None:
+        _check(_lib.lu2mi_randomsetmat(self.handle, n, seed, dA.ptr, lda))
+
+    def lumcolumn_dev(self, n, dA, lda, nb, dpiv=None, dx=None) -> None:
+        _check(_lib.lu2mi_lumcolumn_dev(self.handle, n, dA.ptr, lda, nb,
+                                        dpiv.ptr if dpiv else None, dx.ptr if dx else None))
+
+    def dgemm_dev(self, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, offs=(0, 0, 0)) -> None:
+        _check(_lib.lu2mi_dgemm_dev(self.handle, m, n, k, alpha, dA.ptr + offs[0], lda,
+                                    dB.ptr + offs[1], ldb, beta, dC.ptr + offs[2], ldc))
+
+    def residual_dev(self, n, dA0, lda, dx) -> np.ndarray:
+        out = np.zeros(10)
+        _check(_lib.lu2mi_residual_dev(self.handle, n, dA0.ptr, lda, dx.ptr, out.ctypes.data))
+        return out
+
+    def timing(self) -> dict:
+        out = np.zeros(4)
+        _check(_lib.lu2mi_get_timing(self.handle, out.ctypes.data, 4))
+        return {"gemm_ms": out[0], "gemm_launches": int(out[1]), "gemm_flops": out[2], "wall_ms": out[3]}
+
+
+def solve(a_aug: np.ndarray, nb: int, pivmode: int = PIV_GEPP, ctx: Context | None = None):
+    """Factor lu2's augmented matrix (n x lda, b in column n) on the GPU.
+
+    Returns (x, piv, factored) where factored is lu2's stored factor form.
+    """
+    own = ctx is None
+    ctx = ctx or Context()
+    try:
+        a = np.ascontiguousarray(a_aug, dtype=np.float64)
+        n, lda = a.shape
+        dA = ctx.alloc(a.nbytes)
+        dx = ctx.alloc(8 * n)
+        dp = ctx.alloc(4 * n)
+        dA.upload(a)
+        ctx.set_option(OPT_PIVMODE, pivmode)
+        ctx.lumcolumn_dev(n, dA, lda, nb, dp, dx)
+        x = dx.download(np.float64, n)
+        piv = dp.download(np.int32, n)
+        fac = dA.download(np.float64, n * lda).reshape(n, lda)
+        for b in (dA, dx, dp):
+            b.free()
+        return x, piv, fac
+    finally:
+        if own:
+            ctx.close()

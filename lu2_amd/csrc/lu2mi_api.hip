@@ -1,0 +1,456 @@
+// lu2mi_api.hip -- C ABI of liblu2mi.so.
+//
+// Two groups of entry points:
+//  * lu2mi_* (include/lu2mi.h): device-pointer API used by the GPU-resident
+//    driver, the tests and the benchmark.
+//  * the reference's own link seams (include/lu2mi_seams.h), so that lu2's
+//    nompi objects can link against this library:
+//      gdrdgemm.h        mygdrdgemm, gdrblas_dgemm, gdr_check_and_restart
+//      GRAPE-DR device   gdr_dgemm_, gdr_init, gdr_free, gdrdgemm_set_procname,
+//                        gdrsetboardid, gdrsetnboards, gdrsetforceswapab (+ MPI-only stubs)
+//      lu2lib.h          mydgemm, matmul_for_small_nk, reset_gdr
+//      lu2.c:1122        lumcolumn (GPU-resident factor + solve of a host matrix)
+//    These take host pointers, copy to the device, run the FP64 MFMA kernels
+//    and copy the result back; they keep the reference's calling conventions
+//    and error behaviour (doneflag, known-answer test with re-init, exit on
+//    fatal errors).
+#include "lu2mi_internal.h"
+#include "lu2mi_seams.h"
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+int lu2mi_randomsetmat_impl(lu2mi_ctx *ctx, int n, int seed, double *dA, long lda);
+int lu2mi_residual_impl(lu2mi_ctx *ctx, int n, const double *dA0, long lda, const double *dx, double *out);
+
+#define CTX_OK(ctx)                                                   \
+    do {                                                              \
+        if (!(ctx)) {                                                 \
+            lu2mi_set_error_msg("null context");                      \
+            return LU2MI_ERR_ARG;                                     \
+        }                                                             \
+        LU2MI_CHECK(hipSetDevice((ctx)->device));                     \
+    } while (0)
+
+extern "C" int lu2mi_init(int device, lu2mi_ctx **out)
+{
+    if (!out) return LU2MI_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    LU2MI_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        lu2mi_set_error_msg("device %d not present (%d devices)", device, ndev);
+        return LU2MI_ERR_ARG;
+    }
+    LU2MI_CHECK(hipSetDevice(device));
+    lu2mi_ctx *ctx = (lu2mi_ctx *)calloc(1, sizeof(lu2mi_ctx));
+    if (!ctx) return LU2MI_ERR_NOMEM;
+    ctx->device = device;
+    ctx->pivmode = LU2MI_PIV_COMPAT;
+    ctx->leaf_w = 32;
+    hipDeviceProp_t prop;
+    LU2MI_CHECK(hipGetDeviceProperties(&prop, device));
+    ctx->num_cu = prop.multiProcessorCount;
+    LU2MI_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    LU2MI_CHECK(hipMalloc(&ctx->d_bar, 256));
+    LU2MI_CHECK(hipMemset(ctx->d_bar, 0, 256));
+    LU2MI_CHECK(hipMalloc(&ctx->d_err, 256));
+    LU2MI_CHECK(hipMemset(ctx->d_err, 0, 256));
+    ctx->bar_count = 0;
+    *out = ctx;
+    return LU2MI_OK;
+}
+
+static void free_timer(KernelTimer &kt)
+{
+    for (int e = 0; e < 2 * kt.cap; e++) (void)hipEventDestroy(kt.ev[e]);
+    free(kt.ev);
+    free(kt.flops);
+    kt.ev = nullptr;
+    kt.flops = nullptr;
+    kt.cap = kt.used = 0;
+}
+
+extern "C" int lu2mi_finalize(lu2mi_ctx *ctx)
+{
+    if (!ctx) return LU2MI_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    void *bufs[] = {ctx->d_piv, ctx->d_orig, ctx->d_invd, ctx->d_invd_f, ctx->d_pairs, ctx->d_linv,
+                    ctx->d_tmp, ctx->d_w, ctx->d_slots, ctx->d_bar, ctx->d_err, ctx->d_y,
+                    ctx->h_dev_a, ctx->h_dev_b, ctx->h_dev_c};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    free_timer(ctx->tgemm);
+    (void)hipStreamDestroy(ctx->stream);
+    free(ctx);
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_set_option(lu2mi_ctx *ctx, int opt, int value)
+{
+    if (!ctx) return LU2MI_ERR_ARG;
+    switch (opt) {
+    case LU2MI_OPT_PIVMODE:
+        if (value != LU2MI_PIV_COMPAT && value != LU2MI_PIV_GEPP) return LU2MI_ERR_ARG;
+        ctx->pivmode = value;
+        return LU2MI_OK;
+    case LU2MI_OPT_LEAF_W:
+        if (value != 16 && value != 32) return LU2MI_ERR_ARG;
+        ctx->leaf_w = value;
+        return LU2MI_OK;
+    case LU2MI_OPT_TIMING:
+        ctx->timing_on = value ? 1 : 0;
+        if (ctx->timing_on && ctx->tgemm.cap == 0) {
+            const int cap = 4096;
+            ctx->tgemm.ev = (hipEvent_t *)calloc(2 * cap, sizeof(hipEvent_t));
+            ctx->tgemm.flops = (double *)calloc(cap, sizeof(double));
+            if (!ctx->tgemm.ev || !ctx->tgemm.flops) return LU2MI_ERR_NOMEM;
+            LU2MI_CHECK(hipSetDevice(ctx->device));
+            for (int e = 0; e < 2 * cap; e++) LU2MI_CHECK(hipEventCreate(&ctx->tgemm.ev[e]));
+            ctx->tgemm.cap = cap;
+        }
+        return LU2MI_OK;
+    default:
+        lu2mi_set_error_msg("unknown option %d", opt);
+        return LU2MI_ERR_ARG;
+    }
+}
+
+extern "C" int lu2mi_synchronize(lu2mi_ctx *ctx)
+{
+    CTX_OK(ctx);
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_device_info(lu2mi_ctx *ctx, int *num_cu, char *arch, int arch_len)
+{
+    CTX_OK(ctx);
+    hipDeviceProp_t prop;
+    LU2MI_CHECK(hipGetDeviceProperties(&prop, ctx->device));
+    if (num_cu) *num_cu = prop.multiProcessorCount;
+    if (arch && arch_len > 0) {
+        strncpy(arch, prop.gcnArchName, (size_t)arch_len - 1);
+        arch[arch_len - 1] = 0;
+    }
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_malloc(lu2mi_ctx *ctx, void **dptr, size_t bytes)
+{
+    CTX_OK(ctx);
+    if (!dptr) return LU2MI_ERR_ARG;
+    if (hipMalloc(dptr, bytes) != hipSuccess) {
+        *dptr = nullptr;
+        lu2mi_set_error_msg("hipMalloc(%zu) failed", bytes);
+        return LU2MI_ERR_NOMEM;
+    }
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_free(lu2mi_ctx *ctx, void *dptr)
+{
+    CTX_OK(ctx);
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    if (dptr) LU2MI_CHECK(hipFree(dptr));
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_memcpy_h2d(lu2mi_ctx *ctx, void *d, const void *h, size_t bytes)
+{
+    CTX_OK(ctx);
+    LU2MI_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_memcpy_d2h(lu2mi_ctx *ctx, void *h, const void *d, size_t bytes)
+{
+    CTX_OK(ctx);
+    LU2MI_CHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_memcpy_d2d(lu2mi_ctx *ctx, void *d, const void *s, size_t bytes)
+{
+    CTX_OK(ctx);
+    LU2MI_CHECK(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_randomsetmat(lu2mi_ctx *ctx, int n, int seed, double *dA, long lda)
+{
+    CTX_OK(ctx);
+    return lu2mi_randomsetmat_impl(ctx, n, seed, dA, lda);
+}
+
+extern "C" int lu2mi_lumcolumn_dev(lu2mi_ctx *ctx, int n, double *dA, long lda, int nb, int *dpiv, double *dx)
+{
+    CTX_OK(ctx);
+    return lu2mi_factor_f64(ctx, n, dA, lda, nb, ctx->pivmode, dpiv, dx);
+}
+
+extern "C" int lu2mi_dgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, double alpha, const double *dA, long lda,
+                               const double *dB, long ldb, double beta, double *dC, long ldc)
+{
+    CTX_OK(ctx);
+    if (m < 0 || n < 0 || k < 0) return LU2MI_ERR_ARG;
+    GemmDesc g;
+    g.M = m; g.N = n; g.K = k; g.batch = 1; g.alpha = alpha; g.beta = beta;
+    g.A = dA; g.lda = lda; g.sA = 0; g.B = dB; g.ldb = ldb; g.sB = 0; g.C = dC; g.ldc = ldc; g.sC = 0;
+    g.a_lower = 0;
+    LU2MI_TRY(lu2mi_gemm_f64(g, ctx->stream));
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_residual_dev(lu2mi_ctx *ctx, int n, const double *dA0, long lda, const double *dx, double *out)
+{
+    CTX_OK(ctx);
+    return lu2mi_residual_impl(ctx, n, dA0, lda, dx, out);
+}
+
+extern "C" int lu2mi_get_timing(lu2mi_ctx *ctx, double *out, int nout)
+{
+    CTX_OK(ctx);
+    if (!out || nout < 4) return LU2MI_ERR_ARG;
+    double ms = 0.0, fl = 0.0;
+    KernelTimer &kt = ctx->tgemm;
+    for (int e = 0; e < kt.used; e++) {
+        float m = 0.f;
+        LU2MI_CHECK(hipEventElapsedTime(&m, kt.ev[2 * e], kt.ev[2 * e + 1]));
+        ms += m;
+        fl += kt.flops[e];
+    }
+    out[0] = ms;
+    out[1] = kt.used;
+    out[2] = fl;
+    out[3] = ctx->t_last_ms[3];
+    return LU2MI_OK;
+}
+
+// ===========================================================================
+// Reference-shaped seams (host pointers)
+// ===========================================================================
+static lu2mi_ctx *g_default = nullptr;
+
+lu2mi_ctx *lu2mi_default_ctx(void)
+{
+    if (g_default) return g_default;
+    int dev = 0;
+    const char *e = getenv("LU2MI_DEVICE");
+    if (e) dev = atoi(e);
+    if (lu2mi_init(dev, &g_default) != LU2MI_OK) {
+        fprintf(stderr, "lu2mi: cannot initialise device %d: %s\n", dev, lu2mi_last_error());
+        exit(-1);
+    }
+    const char *pm = getenv("LU2MI_PIVOT");
+    if (pm && strcmp(pm, "gepp") == 0) g_default->pivmode = LU2MI_PIV_GEPP;
+    return g_default;
+}
+
+static void fatal(const char *where)
+{
+    fprintf(stderr, "lu2mi: %s failed: %s\n", where, lu2mi_last_error());
+    exit(-1);
+}
+
+// Row-major C = alpha*A*B + beta*C with host matrices, on the GPU.
+static int host_gemm_rm(int m, int n, int k, double alpha, const double *a, long na, const double *b, long nb,
+                        double beta, double *c, long nc)
+{
+    if (m <= 0 || n <= 0) return LU2MI_OK;
+    lu2mi_ctx *ctx = lu2mi_default_ctx();
+    LU2MI_CHECK(hipSetDevice(ctx->device));
+    const size_t kk = k > 0 ? (size_t)k : 1;
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->h_dev_a, &ctx->h_dev_a_bytes, sizeof(double) * (size_t)m * kk));
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->h_dev_b, &ctx->h_dev_b_bytes, sizeof(double) * kk * (size_t)n));
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->h_dev_c, &ctx->h_dev_c_bytes, sizeof(double) * (size_t)m * (size_t)n));
+    double *dA = (double *)ctx->h_dev_a, *dB = (double *)ctx->h_dev_b, *dC = (double *)ctx->h_dev_c;
+    hipStream_t s = ctx->stream;
+    if (k > 0) {
+        LU2MI_CHECK(hipMemcpy2DAsync(dA, sizeof(double) * k, a, sizeof(double) * na, sizeof(double) * k, m,
+                                     hipMemcpyHostToDevice, s));
+        LU2MI_CHECK(hipMemcpy2DAsync(dB, sizeof(double) * n, b, sizeof(double) * nb, sizeof(double) * n, k,
+                                     hipMemcpyHostToDevice, s));
+    }
+    if (beta != 0.0)
+        LU2MI_CHECK(hipMemcpy2DAsync(dC, sizeof(double) * n, c, sizeof(double) * nc, sizeof(double) * n, m,
+                                     hipMemcpyHostToDevice, s));
+    GemmDesc g;
+    g.M = m; g.N = n; g.K = k > 0 ? k : 0; g.batch = 1; g.alpha = alpha; g.beta = beta;
+    g.A = dA; g.lda = kk; g.sA = 0; g.B = dB; g.ldb = n; g.sB = 0; g.C = dC; g.ldc = n; g.sC = 0;
+    g.a_lower = 0;
+    LU2MI_TRY(lu2mi_gemm_f64(g, s));
+    LU2MI_CHECK(hipMemcpy2DAsync(c, sizeof(double) * nc, dC, sizeof(double) * n, sizeof(double) * n, m,
+                                 hipMemcpyDeviceToHost, s));
+    LU2MI_CHECK(hipStreamSynchronize(s));
+    return LU2MI_OK;
+}
+
+// gdrdgemm.c:155-199 mygdrdgemm: row-major C = alpha*A*B + beta*C.
+extern "C" void mygdrdgemm(int m, int n, int k, double alpha, double *a, int na, double *b, int nb, double beta,
+                           double *c, int nc)
+{
+    if (host_gemm_rm(m, n, k, alpha, a, na, b, nb, beta, c, nc) != LU2MI_OK) fatal("mygdrdgemm");
+}
+
+// gdrdgemm.c:64-120 gdrblas_dgemm (CBLAS enums as int: RowMajor=101,
+// ColMajor=102, NoTrans=111, Trans=112).
+extern "C" void gdrblas_dgemm(int order, int transa, int transb, int M, int N, int K, double alpha, const double *A,
+                              int lda, const double *B, int ldb, double beta, double *C, int ldc)
+{
+    // reduce to row-major NoTrans: col-major C = row-major C^T = B^T A^T
+    const bool rowmajor = (order == LU2MI_CBLAS_ROWMAJOR);
+    int m = rowmajor ? M : N, n = rowmajor ? N : M;
+    const double *a = rowmajor ? A : B, *b = rowmajor ? B : A;
+    int la = rowmajor ? lda : ldb, lb = rowmajor ? ldb : lda;
+    bool ta = (rowmajor ? transa : transb) != LU2MI_CBLAS_NOTRANS;
+    bool tb = (rowmajor ? transb : transa) != LU2MI_CBLAS_NOTRANS;
+    std::vector<double> ta_buf, tb_buf;
+    if (ta) {  // stored K x m (row-major, ld la); need m x K
+        ta_buf.resize((size_t)m * K);
+        for (int r = 0; r < K; r++)
+            for (int q = 0; q < m; q++) ta_buf[(size_t)q * K + r] = a[(size_t)r * la + q];
+        a = ta_buf.data();
+        la = K;
+    }
+    if (tb) {  // stored n x K; need K x n
+        tb_buf.resize((size_t)K * n);
+        for (int r = 0; r < n; r++)
+            for (int q = 0; q < K; q++) tb_buf[(size_t)q * n + r] = b[(size_t)r * lb + q];
+        b = tb_buf.data();
+        lb = n;
+    }
+    if (host_gemm_rm(m, n, K, alpha, a, la, b, lb, beta, C, ldc) != LU2MI_OK) fatal("gdrblas_dgemm");
+}
+
+// GRAPE-DR device library entry (Fortran ABI, column-major), called by
+// gdrdgemm.c:109-110,185-186.  Only the NoTrans forms are used by lu2; for
+// the others doneflag stays 0 and the caller falls back to its CBLAS
+// (gdrdgemm.c:113-116,191-194).
+extern "C" void gdr_dgemm_(int *nota, int *notb, int *m, int *n, int *k, double *alpha, double *beta, int *lda,
+                           int *ldb, int *ldc, const double *A, const double *B, double *C, int *doneflag)
+{
+    *doneflag = 0;
+    if (!*nota || !*notb) return;
+    // column-major C(m x n) = A(m x k) B(k x n)  <=>  row-major C^T = B^T A^T
+    if (host_gemm_rm(*n, *m, *k, *alpha, B, *ldb, A, *lda, *beta, C, *ldc) == LU2MI_OK) *doneflag = 1;
+}
+
+static int g_boardid = 0, g_nboards = 1;
+extern "C" void gdr_init(void) { (void)lu2mi_default_ctx(); }
+extern "C" void gdr_free(void)
+{
+    if (g_default) {
+        lu2mi_finalize(g_default);
+        g_default = nullptr;
+    }
+}
+extern "C" void gdrdgemm_set_procname(int procid) { (void)procid; }
+extern "C" void gdrsetboardid(int boardid) { g_boardid = boardid; }
+extern "C" void gdrsetnboards(int nboards) { g_nboards = nboards; }
+extern "C" void gdrsetforceswapab(void) {}
+extern "C" void gdrresetforceswapab(void) {}
+extern "C" void gdrsetskipsendjmat(void) {}
+extern "C" void gdrresetskipsendjmat(void) {}
+extern "C" void gdrdgemm_set_stress_factor(int x) { (void)x; }
+extern "C" void set_matmul_msg_level(int level) { (void)level; }
+
+// gdrdgemm.c:17-58 known-answer test: I(2048^2) * B == B exactly, B[i][j] =
+// 2048*i + j, re-initialising the device between attempts.
+extern "C" void gdr_check_and_restart(double a[][LU2MI_NMAT], double b[][LU2MI_NMAT], double c[][LU2MI_NMAT])
+{
+    static int initialized = 0;
+    if (initialized) return;
+    const int N = LU2MI_NMAT;
+    for (int attempt = 0; attempt < 8; attempt++) {
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) {
+                a[i][j] = (i == j) ? 1.0 : 0.0;
+                b[i][j] = (double)i * N + j;
+                c[i][j] = 0.0;
+            }
+        mygdrdgemm(N, N, N, 1.0, (double *)a, N, (double *)b, N, 0.0, (double *)c, N);
+        long err = 0;
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) err += (b[i][j] != c[i][j]);
+        if (err == 0) {
+            fprintf(stderr, "gdr_check_and_restart passed %d\n", attempt);
+            initialized = 1;
+            return;
+        }
+        fprintf(stderr, "gdr_check_and_restart, err=%ld try=%d\n", err, attempt + 1);
+        gdr_free();
+        gdr_init();
+    }
+    fprintf(stderr, "gdr_check_and_restart: device failed the known-answer test\n");
+    exit(-1);
+}
+
+// lu2lib.c:2079-2135 mydgemm: every shape goes to the GPU (the reference
+// routes K>=512 to GRAPE-DR and the rest to CBLAS or SSE2 kernels).
+extern "C" void mydgemm(int m, int n, int k, double alpha, double *a, int na, double *b, int nb, double beta,
+                        double *c, int nc)
+{
+    if (host_gemm_rm(m, n, k, alpha, a, na, b, nb, beta, c, nc) != LU2MI_OK) fatal("mydgemm");
+}
+
+// lu2lib.c:1984-2075 matmul_for_small_nk: C[m x n] -= A[m x kk] * B[kk x n].
+extern "C" void matmul_for_small_nk(int n1, double *a, int n2, double *b, int n3, double *c, int m, int kk, int n)
+{
+    if (host_gemm_rm(m, n, kk, -1.0, a, n1, b, n2, 1.0, c, n3) != LU2MI_OK) fatal("matmul_for_small_nk");
+}
+
+// lu2lib.c:2138-2168 reset_gdr: device self-test and warm-up.  The
+// reference requires nb >= 2048 for GRAPE-DR (lu2lib.c:2142-2145); the GPU has
+// no such limit.  Scratch for the test is the library's own.
+extern "C" void reset_gdr(int m, double *a, int nb, double *awork, int n)
+{
+    (void)m; (void)a; (void)nb; (void)awork; (void)n;
+    (void)lu2mi_default_ctx();
+    const size_t N = LU2MI_NMAT;
+    double *s = (double *)malloc(sizeof(double) * N * N * 3);
+    if (!s) {
+        fprintf(stderr, "reset_gdr: out of host memory\n");
+        exit(-1);
+    }
+    gdr_check_and_restart((double(*)[LU2MI_NMAT])s, (double(*)[LU2MI_NMAT])(s + N * N),
+                          (double(*)[LU2MI_NMAT])(s + 2 * N * N));
+    free(s);
+}
+
+// lu2.c:1122-1144 lumcolumn on a host matrix a[n][n+16] (b in column n):
+// copy to the GPU, factor and solve there, copy the factored matrix back,
+// x into b[], and the last block's pivots into pv[0..m) as the reference
+// leaves them.  The pivot rule is COMPAT unless LU2MI_PIVOT=gepp.  The
+// reference driver always passes recursive=1 (lu2.c:1292); 0 is rejected.
+extern "C" void lumcolumn(int n, double *a, double *b, int m, double *awork, int *pv, int recursive)
+{
+    (void)awork;
+    if (!recursive) {
+        fprintf(stderr, "lumcolumn: only the recursive panel (recursive=1) is provided\n");
+        exit(-1);
+    }
+    lu2mi_ctx *ctx = lu2mi_default_ctx();
+    const long lda = (long)n + 16;
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)lda;
+    double *dA = nullptr, *dx = nullptr;
+    int *dp = nullptr;
+    if (lu2mi_malloc(ctx, (void **)&dA, bytes) || lu2mi_malloc(ctx, (void **)&dx, sizeof(double) * n) ||
+        lu2mi_malloc(ctx, (void **)&dp, sizeof(int) * n))
+        fatal("lumcolumn allocation");
+    if (lu2mi_memcpy_h2d(ctx, dA, a, bytes)) fatal("lumcolumn copy-in");
+    if (lu2mi_factor_f64(ctx, n, dA, lda, m, ctx->pivmode, dp, dx)) fatal("lumcolumn factorization");
+    std::vector<int> piv(n);
+    if (lu2mi_memcpy_d2h(ctx, a, dA, bytes) || lu2mi_memcpy_d2h(ctx, b, dx, sizeof(double) * n) ||
+        lu2mi_memcpy_d2h(ctx, piv.data(), dp, sizeof(int) * n))
+        fatal("lumcolumn copy-out");
+    for (int k = 0; k < m && k < n; k++) pv[k] = piv[n - m + k];
+    lu2mi_free(ctx, dA);
+    lu2mi_free(ctx, dx);
+    lu2mi_free(ctx, dp);
+}

@@ -1,0 +1,92 @@
+// lu2mi_internal.h -- shared declarations of the MI355X LU library (not public).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "lu2mi.h"
+
+#define LU2MI_CHECK(call)                                                          \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            lu2mi_set_error_msg("%s:%d %s -> %s", __FILE__, __LINE__, #call,       \
+                                hipGetErrorString(e_));                            \
+            return LU2MI_ERR_HIP;                                                  \
+        }                                                                          \
+    } while (0)
+
+#define LU2MI_TRY(expr)                                                            \
+    do {                                                                           \
+        int rc_ = (expr);                                                          \
+        if (rc_ != LU2MI_OK) return rc_;                                           \
+    } while (0)
+
+void lu2mi_set_error_msg(const char *fmt, ...);
+
+// ---------------------------------------------------------------------------
+// GEMM: C = alpha * A * B + beta * C, all row-major (lu2lib.c:2079 shape).
+// Batched over gridDim.y with element strides sA/sB/sC.  When a_lower is set,
+// A is lower triangular (columns > row are zero) and each row tile only runs
+// the K range it needs.
+struct GemmDesc {
+    int M, N, K, batch;
+    double alpha, beta;
+    const void *A; long lda; long sA;
+    const void *B; long ldb; long sB;
+    void *C; long ldc; long sC;
+    int a_lower;
+};
+int lu2mi_gemm_f64(const GemmDesc &g, hipStream_t s);
+int lu2mi_gemm_f32(const GemmDesc &g, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Per-context device workspace for the GPU-resident factorization.
+struct KernelTimer {
+    // events around every trailing-update GEMM launch, read after sync
+    hipEvent_t *ev;      // 2 * cap
+    double *flops;       // cap
+    int cap, used;
+};
+
+struct lu2mi_ctx {
+    int device;
+    hipStream_t stream;
+    int num_cu;
+    // workspace sized for n_cap rows, nb_cap block
+    int n_cap, nb_cap;
+    int *d_piv;            // n   global pivot rows
+    int *d_orig;           // n   block-start row id of the data held by each row
+    double *d_invd;        // n   1/d per row (fp64 path)
+    float *d_invd_f;       // n   1/d per row (fp32 path)
+    int *d_pairs;          // 4*nb_cap (dst, src) lists
+    void *d_linv;          // nb*nb inverse of the unit-lower diagonal block
+    void *d_tmp;           // nb*nb scratch for the inverse recursion
+    void *d_w;             // nb*(n+1) right-part staging (awork of lu2.c:1263)
+    size_t w_bytes;
+    void *d_slots;         // leaf kernel exchange slots
+    size_t slots_bytes;
+    unsigned long long *d_bar;  // leaf grid-barrier counter (monotonic)
+    unsigned long long bar_count;
+    int *d_err;            // device error flag
+    double *d_y;           // back-substitution vector scratch
+    // options
+    int pivmode;
+    int leaf_w;
+    int timing_on;
+    KernelTimer tgemm;
+    double t_last_ms[8];
+    // host staging buffers for the host-memory seams (mygdrdgemm, lumcolumn)
+    void *h_dev_a; size_t h_dev_a_bytes;
+    void *h_dev_b; size_t h_dev_b_bytes;
+    void *h_dev_c; size_t h_dev_c_bytes;
+};
+
+int lu2mi_ctx_reserve(lu2mi_ctx *ctx, int n, int nb, int elem_bytes);
+int lu2mi_factor_f64(lu2mi_ctx *ctx, int n, double *dA, long lda, int nb, int pivmode,
+                     int *dpiv_out, double *dx);
+int lu2mi_factor_f32(lu2mi_ctx *ctx, int n, float *dA, long lda, int nb, int pivmode,
+                     int *dpiv_out, float *dx);
+int lu2mi_dev_buf(void **p, size_t *cur, size_t need);
+
+// default context used by the lu2-shaped (reference-signature) entry points
+lu2mi_ctx *lu2mi_default_ctx(void);

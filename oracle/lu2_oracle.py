@@ -1,0 +1,123 @@
+"""ctypes bindings to the CPU oracle (oracle/_build/liblu2oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the lu2_amd package.  See
+lu2_oracle.h for the functions and the reference lines they restate.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liblu2oracle.so")
+PIV_COMPAT = 0
+PIV_GEPP = 1
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        c_int, c_long, c_double = ctypes.c_int, ctypes.c_long, ctypes.c_double
+        L.lu2o_randomsetmat.argtypes = [c_int, c_int, _dp, c_long]
+        L.lu2o_dgemm.argtypes = [c_int, c_int, c_int, c_double, _dp, c_long, _dp, c_long, c_double,
+                                 _dp, c_long]
+        L.lu2o_lumcolumn.argtypes = [c_int, _dp, c_long, c_int, c_int, _ip, _dp]
+        L.lu2o_lumcolumn.restype = c_int
+        L.lu2o_gepp_unblocked.argtypes = L.lu2o_lumcolumn.argtypes
+        L.lu2o_gepp_unblocked.restype = c_int
+        L.lu2o_backward_sub.argtypes = [c_int, _dp, c_long, _dp]
+        L.lu2o_emax.argtypes = [c_int, _dp, c_long, _dp]
+        L.lu2o_emax.restype = c_double
+        L.lu2o_residuals.argtypes = [c_int, _dp, c_long, _dp, _dp]
+        L.lu2o_factor_solve.argtypes = [c_int, _dp, c_long, c_int, _ip, _dp]
+        L.lu2o_kat_identity.argtypes = [c_int]
+        L.lu2o_kat_identity.restype = ctypes.c_long
+        _lib = L
+    return _lib
+
+
+def randomsetmat(n: int, seed: int, lda: int | None = None) -> np.ndarray:
+    """lu2.c:70-85: n x lda augmented matrix, b in column n, pad columns 0."""
+    lda = n + 16 if lda is None else lda
+    a = np.zeros((n, lda))
+    lib().lu2o_randomsetmat(n, seed, a, lda)
+    return a
+
+
+def lumcolumn(a: np.ndarray, nb: int, pivmode: int):
+    """Blocked recursive LU (lu2.c:1122).  Returns (x, piv, factored copy)."""
+    f = np.array(a, dtype=np.float64, copy=True, order="C")
+    n, lda = f.shape
+    piv = np.zeros(n, np.int32)
+    x = np.zeros(n)
+    rc = lib().lu2o_lumcolumn(n, f, lda, nb, pivmode, piv, x)
+    if rc:
+        raise ValueError(f"lu2o_lumcolumn rc={rc}")
+    return x, piv, f
+
+
+def gepp_unblocked(a: np.ndarray, nb: int, pivmode: int):
+    f = np.array(a, dtype=np.float64, copy=True, order="C")
+    n, lda = f.shape
+    piv = np.zeros(n, np.int32)
+    x = np.zeros(n)
+    rc = lib().lu2o_gepp_unblocked(n, f, lda, nb, pivmode, piv, x)
+    if rc:
+        raise ValueError(f"lu2o_gepp_unblocked rc={rc}")
+    return x, piv, f
+
+
+def dgemm(alpha, a, b, beta, c):
+    c = np.array(c, dtype=np.float64, copy=True, order="C")
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    m, k = a.shape
+    n = b.shape[1]
+    lib().lu2o_dgemm(m, n, k, alpha, a, max(k, 1), b, max(n, 1), beta, c, max(n, 1))
+    return c
+
+
+def backward_sub(f: np.ndarray) -> np.ndarray:
+    n = f.shape[0]
+    x = np.zeros(n)
+    lib().lu2o_backward_sub(n, np.ascontiguousarray(f), f.shape[1], x)
+    return x
+
+
+def residuals(a0: np.ndarray, x: np.ndarray) -> np.ndarray:
+    n, lda = a0.shape
+    out = np.zeros(10)
+    lib().lu2o_residuals(n, np.ascontiguousarray(a0), lda, np.ascontiguousarray(x), out)
+    return out
+
+
+def emax(a0: np.ndarray, x: np.ndarray) -> float:
+    n, lda = a0.shape
+    return lib().lu2o_emax(n, np.ascontiguousarray(a0), lda, np.ascontiguousarray(x))
+
+
+def factor_solve(f: np.ndarray, nb: int, piv: np.ndarray, r: np.ndarray) -> np.ndarray:
+    n, lda = f.shape
+    out = np.array(r, dtype=np.float64, copy=True)
+    lib().lu2o_factor_solve(n, np.ascontiguousarray(f), lda, nb, np.ascontiguousarray(piv, np.int32), out)
+    return out
+
+
+def kat_identity(nmat: int) -> int:
+    return lib().lu2o_kat_identity(nmat)
