@@ -1,0 +1,70 @@
+"""Build liblu2mi.so (HIP, gfx950), the lu2mi driver, and the CPU oracle.
+
+    python -m lu2_amd.build          # or __graft_entry__.build()
+
+Everything is built in-tree so the shared objects travel with the source.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+SOURCES = ["lu2mi_gemm.hip", "lu2mi_lu.hip", "lu2mi_api.hip"]
+LIB = os.path.join(HERE, "liblu2mi.so")
+DRIVER = os.path.join(HERE, "lu2mi")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# packed-FP32 instructions are kept out of the generated code on purpose
+# MFMA accumulators in ordinary VGPRs: with AGPR accumulators the compiler
+# copies every accumulator across the K-loop back-edge.
+HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-mllvm", "-amdgpu-mfma-vgpr-form=true"]
+
+
+def _run(cmd, **kw):
+    r = subprocess.run(cmd, capture_output=True, text=True, **kw)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd[:3])} ...")
+    return r
+
+
+def build_library() -> str:
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(CSRC, "_build", src.replace(".hip", ".o"))
+        os.makedirs(os.path.dirname(obj), exist_ok=True)
+        _run([HIPCC, *HIPFLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+              "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_driver() -> str:
+    _run(["gcc", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(HERE, "lu2mi_main.c"), "-o", DRIVER, "-L", HERE, "-llu2mi",
+          "-Wl,-rpath,$ORIGIN"])
+    return DRIVER
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def build_all() -> None:
+    build_library()
+    build_driver()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB, DRIVER)

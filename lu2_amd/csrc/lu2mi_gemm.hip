@@ -48,25 +48,21 @@ __device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int
 }  // namespace
 
 // ---------------------------------------------------------------------------
-template <bool VEC>
-__global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, int tiles_m, int tiles_n)
-{
-    __shared__ double As[2][BM * ALD];
-    __shared__ double Bs[2][BK * BN];
+// Epilogue modes: EPI_GENERAL  C = alpha*acc + beta*C
+//                 EPI_SUB      C = C - acc        (alpha = -1, beta = 1: the LU updates)
+//                 EPI_SET      C = acc            (alpha = 1, beta = 0: the TRSM products)
+enum { EPI_GENERAL = 0, EPI_SUB = 1, EPI_SET = 2 };
 
-    const int total = tiles_m * tiles_n;
-    const int tile = xcd_remap(blockIdx.x, total);
-    int tm, tn;
-    tile_coords(tile, tiles_m, tiles_n, tm, tn);
-    const int m0 = tm * BM, n0 = tn * BN;
-    const long bz = blockIdx.y;
-    const double *__restrict__ A = (const double *)g.A + bz * g.sA;
-    const double *__restrict__ B = (const double *)g.B + bz * g.sB;
-    double *__restrict__ C = (double *)g.C + bz * g.sC;
+template <bool EDGE, int EPI>
+__device__ __forceinline__ void gemm_f64_tile(const GemmDesc &g, const double *__restrict__ A,
+                                              const double *__restrict__ B, double *__restrict__ C,
+                                              int m0, int n0, int K, double (*As)[BM * ALD],
+                                              double (*Bs)[BK * BN])
+{
     const int M = g.M, N = g.N;
-    const int K = g.a_lower ? min(g.K, m0 + BM) : g.K;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w >> 1, wn = w & 1;
+    const long lda = g.lda, ldb = g.ldb;
 
     double2 ra[4], rb[4];
     auto load_tiles = [&](int k0) {
@@ -74,31 +70,28 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, int tiles_m, i
         for (int it = 0; it < 4; it++) {
             const int idx = t + it * NT;
             const int ar = idx >> 3, ac = (idx & 7) * 2;
-            const int gr = m0 + ar, gc = k0 + ac;
-            double2 v = {0.0, 0.0};
-            if (gr < M) {
-                const double *p = A + (long)gr * g.lda + gc;
-                if (VEC && gc + 1 < K) {
-                    v = *(const double2 *)p;
-                } else {
+            const int br = idx >> 6, bc = (idx & 63) * 2;
+            if (!EDGE) {
+                ra[it] = *(const double2 *)(A + (long)(m0 + ar) * lda + k0 + ac);
+                rb[it] = *(const double2 *)(B + (long)(k0 + br) * ldb + n0 + bc);
+            } else {
+                const int gr = m0 + ar, gc = k0 + ac;
+                double2 v = {0.0, 0.0};
+                if (gr < M) {
+                    const double *p = A + (long)gr * lda + gc;
                     if (gc < K) v.x = p[0];
                     if (gc + 1 < K) v.y = p[1];
                 }
-            }
-            ra[it] = v;
-            const int br = idx >> 6, bc = (idx & 63) * 2;
-            const int gk = k0 + br, gn = n0 + bc;
-            double2 u = {0.0, 0.0};
-            if (gk < K) {
-                const double *q = B + (long)gk * g.ldb + gn;
-                if (VEC && gn + 1 < N) {
-                    u = *(const double2 *)q;
-                } else {
+                ra[it] = v;
+                const int gk = k0 + br, gn = n0 + bc;
+                double2 u = {0.0, 0.0};
+                if (gk < K) {
+                    const double *q = B + (long)gk * ldb + gn;
                     if (gn < N) u.x = q[0];
                     if (gn + 1 < N) u.y = q[1];
                 }
+                rb[it] = u;
             }
-            rb[it] = u;
         }
     };
     auto store_tiles = [&](int buf) {
@@ -119,35 +112,40 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, int tiles_m, i
 #pragma unroll
         for (int nj = 0; nj < 4; nj++) acc[mi][nj] = (d4_t){0.0, 0.0, 0.0, 0.0};
 
+    auto compute = [&](int cur) {
+        const double *as = As[cur];
+        const double *bs = Bs[cur];
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double a[4], b[4];
+#pragma unroll
+            for (int mi = 0; mi < 4; mi++)
+                a[mi] = as[(wm * 64 + mi * 16 + (lane & 15)) * ALD + kk + (lane >> 4)];
+#pragma unroll
+            for (int nj = 0; nj < 4; nj++)
+                b[nj] = bs[(kk + (lane >> 4)) * BN + wn * 64 + nj * 16 + (lane & 15)];
+#pragma unroll
+            for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+                for (int nj = 0; nj < 4; nj++)
+                    acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b[nj], acc[mi][nj], 0, 0, 0);
+        }
+    };
+
     const int nk = (K + BK - 1) / BK;
     if (nk > 0) {
         load_tiles(0);
         store_tiles(0);
         __syncthreads();
-        for (int kt = 0; kt < nk; kt++) {
+        // steady state: no conditionals around the loads
+        for (int kt = 0; kt < nk - 1; kt++) {
             const int cur = kt & 1;
-            if (kt + 1 < nk) load_tiles((kt + 1) * BK);
-            const double *as = As[cur];
-            const double *bs = Bs[cur];
-#pragma unroll
-            for (int kk = 0; kk < BK; kk += 4) {
-                double a[4], b[4];
-#pragma unroll
-                for (int mi = 0; mi < 4; mi++)
-                    a[mi] = as[(wm * 64 + mi * 16 + (lane & 15)) * ALD + kk + (lane >> 4)];
-#pragma unroll
-                for (int nj = 0; nj < 4; nj++)
-                    b[nj] = bs[(kk + (lane >> 4)) * BN + wn * 64 + nj * 16 + (lane & 15)];
-#pragma unroll
-                for (int mi = 0; mi < 4; mi++)
-#pragma unroll
-                    for (int nj = 0; nj < 4; nj++)
-                        acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b[nj],
-                                                                          acc[mi][nj], 0, 0, 0);
-            }
-            if (kt + 1 < nk) store_tiles(cur ^ 1);
+            load_tiles((kt + 1) * BK);
+            compute(cur);
+            store_tiles(cur ^ 1);
             __syncthreads();
         }
+        compute((nk - 1) & 1);
     }
 
     const double alpha = g.alpha, beta = g.beta;
@@ -159,12 +157,35 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, int tiles_m, i
             for (int r = 0; r < 4; r++) {
                 const int row = m0 + wm * 64 + mi * 16 + (lane >> 4) + 4 * r;
                 const int col = n0 + wn * 64 + nj * 16 + (lane & 15);
-                if (row < M && col < N) {
+                if (!EDGE || (row < M && col < N)) {
                     double *cp = C + (long)row * g.ldc + col;
                     const double v = acc[mi][nj][r];
-                    *cp = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * *cp);
+                    if (EPI == EPI_SUB) *cp = *cp - v;
+                    else if (EPI == EPI_SET) *cp = v;
+                    else *cp = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * *cp);
                 }
             }
+}
+
+template <bool VEC, int EPI, int TAG>
+__global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, int tiles_m, int tiles_n)
+{
+    __shared__ double As[2][BM * ALD];
+    __shared__ double Bs[2][BK * BN];
+
+    const int total = tiles_m * tiles_n;
+    const int tile = xcd_remap(blockIdx.x, total);
+    int tm, tn;
+    tile_coords(tile, tiles_m, tiles_n, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const long bz = blockIdx.y;
+    const double *A = (const double *)g.A + bz * g.sA;
+    const double *B = (const double *)g.B + bz * g.sB;
+    double *C = (double *)g.C + bz * g.sC;
+    const int K = g.a_lower ? min(g.K, m0 + BM) : g.K;
+    const bool interior = VEC && (m0 + BM <= g.M) && (n0 + BN <= g.N) && (K % BK == 0);
+    if (interior) gemm_f64_tile<false, EPI>(g, A, B, C, m0, n0, K, As, Bs);
+    else gemm_f64_tile<true, EPI>(g, A, B, C, m0, n0, K, As, Bs);
 }
 
 // ---------------------------------------------------------------------------
@@ -313,8 +334,18 @@ static int launch_gemm(const GemmDesc &g, hipStream_t s, bool is_f64)
                      (g.sA % 2 == 0) && (g.sB % 2 == 0);
     dim3 grid((unsigned)total, (unsigned)g.batch), block(NT);
     if (is_f64) {
-        if (vec) hipLaunchKernelGGL(gemm_f64_kernel<true>, grid, block, 0, s, g, tiles_m, tiles_n);
-        else hipLaunchKernelGGL(gemm_f64_kernel<false>, grid, block, 0, s, g, tiles_m, tiles_n);
+        const int epi = (g.alpha == -1.0 && g.beta == 1.0) ? EPI_SUB
+                        : (g.alpha == 1.0 && g.beta == 0.0) ? EPI_SET : EPI_GENERAL;
+        if (vec && epi == EPI_SUB && g.tag == 1)
+            hipLaunchKernelGGL((gemm_f64_kernel<true, EPI_SUB, 1>), grid, block, 0, s, g, tiles_m, tiles_n);
+        else if (vec && epi == EPI_SUB)
+            hipLaunchKernelGGL((gemm_f64_kernel<true, EPI_SUB, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
+        else if (vec && epi == EPI_SET)
+            hipLaunchKernelGGL((gemm_f64_kernel<true, EPI_SET, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
+        else if (vec)
+            hipLaunchKernelGGL((gemm_f64_kernel<true, EPI_GENERAL, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
+        else
+            hipLaunchKernelGGL((gemm_f64_kernel<false, EPI_GENERAL, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
     } else {
         const bool vecf = ((uintptr_t)g.A % 8 == 0) && ((uintptr_t)g.B % 8 == 0) &&
                           (g.lda % 2 == 0) && (g.ldb % 2 == 0) && (g.sA % 2 == 0) && (g.sB % 2 == 0);

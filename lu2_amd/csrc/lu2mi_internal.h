@@ -29,12 +29,13 @@ void lu2mi_set_error_msg(const char *fmt, ...);
 // A is lower triangular (columns > row are zero) and each row tile only runs
 // the K range it needs.
 struct GemmDesc {
-    int M, N, K, batch;
-    double alpha, beta;
-    const void *A; long lda; long sA;
-    const void *B; long ldb; long sB;
-    void *C; long ldc; long sC;
-    int a_lower;
+    int M = 0, N = 0, K = 0, batch = 1;
+    double alpha = 1.0, beta = 0.0;
+    const void *A = nullptr; long lda = 0; long sA = 0;
+    const void *B = nullptr; long ldb = 0; long sB = 0;
+    void *C = nullptr; long ldc = 0; long sC = 0;
+    int a_lower = 0;
+    int tag = 0;   // 1: trailing update (separate kernel symbol for profiling)
 };
 int lu2mi_gemm_f64(const GemmDesc &g, hipStream_t s);
 int lu2mi_gemm_f32(const GemmDesc &g, hipStream_t s);

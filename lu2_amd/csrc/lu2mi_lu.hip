@@ -86,6 +86,7 @@ __global__ __launch_bounds__(256) void randomsetmat_kernel(double *A, long lda, 
 namespace {
 constexpr int SLOT_WORDS = 40;  // 3 header words + up to 32 row values, padded
 constexpr int NTL = 256;        // threads per workgroup in the leaf kernel
+constexpr int LEAF_GMAX = 128;  // most workgroups a leaf launch may use
 
 __device__ __forceinline__ void st_relaxed(unsigned long long *p, unsigned long long v)
 {
@@ -106,20 +107,19 @@ __device__ __forceinline__ void grid_barrier(unsigned long long *bar, unsigned l
 {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
         __hip_atomic_fetch_add(bar, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         unsigned long spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(2);
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
             if (((++spins) & 1023) == 0) {
                 if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (spins > (1UL << 24)) {
+                if (spins > (1UL << 25)) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
@@ -142,7 +142,7 @@ struct LeafArgs {
     int *pair_dst, *pair_src;     // 2W pairs for the panel row permutation
     T *linv;                      // nb x nb inverse of D^-1 L11 (this leaf's block)
     int ldl;
-    unsigned long long *slots;    // 2 * (G+1) * SLOT_WORDS
+    unsigned long long *slots;    // 2 * (G+1) * SLOT_WORDS, G <= LEAF_GMAX
     unsigned long long *bar;
     unsigned long long bar_base;
     int *err;
@@ -152,7 +152,8 @@ template <typename T, int W, int RPT>
 __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
 {
     __shared__ T stage[NTL * (W + 1)];
-    __shared__ T piv_row[W], cur_row[W];
+    // every workgroup's slot (header + row) plus the current-row slot
+    __shared__ unsigned long long xch[(LEAF_GMAX + 1) * (W + 3)];
     __shared__ double red_v[NTL / 64];
     __shared__ int red_r[NTL / 64], red_g[NTL / 64];
     __shared__ int s_win[8];
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
     int lorig[RPT], borig[RPT], rq[RPT];
 
     // coalesced load of the rows through LDS, one 256-row layer at a time
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int q = 0; q < RPT; q++) {
         const long base = (long)c + (long)blockIdx.x * NTL + (long)q * TT;
         rq[q] = (int)min((long)INT_MAX, base + t);
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
             stage[rr * (W + 1) + kk] = (gr < n) ? A[gr * a.lda + c + kk] : T(0);
         }
         __syncthreads();
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int k = 0; k < W; k++) row[q][k] = stage[t * (W + 1) + k];
         __syncthreads();
         lorig[q] = rq[q];
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
     }
 
     unsigned long long *const slots = a.slots;
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int jl = 0; jl < W; jl++) {
         const int j = c + jl;
         const int par = jl & 1;
@@ -195,11 +196,13 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
         // ---- local candidate: first max |x| over owned rows r >= j
         double bv = -1.0;
         int br = INT_MAX;
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int q = 0; q < RPT; q++) {
             const int r = rq[q];
             if (r >= j && r < n) {
-                const T x = odd_rule ? row[q][jl > 0 ? jl - 1 : 0] : row[q][jl];
+                const T xa = row[q][jl];
+                const T xb = row[q][jl > 0 ? jl - 1 : 0];
+                const T x = odd_rule ? xb : xa;
                 const double v = fabs((double)x);
                 if (v > bv) { bv = v; br = r; }
             }
@@ -218,41 +221,59 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
             for (int w2 = 1; w2 < NTL / 64; w2++)
                 if (better(red_v[w2], red_r[w2], v, r)) { v = red_v[w2]; r = red_r[w2]; }
             s_win[0] = r;
-            unsigned long long *sl = slots + ((long)par * (G + 1) + blockIdx.x) * SLOT_WORDS;
-            st_relaxed(sl + 0, dbits(v));
-            st_relaxed(sl + 1, (unsigned long long)(unsigned)r);
+            red_v[0] = v;
         }
         __syncthreads();
         {
+            // publish: the owner of the workgroup's candidate row writes the
+            // slot (value, row, origins, W values); workgroup 0's thread jl
+            // writes the current row j.  Writers fence before the barrier.
             const int wr = s_win[0];
+            const double wv = red_v[0];
             unsigned long long *sl = slots + ((long)par * (G + 1) + blockIdx.x) * SLOT_WORDS;
-#pragma unroll
+            bool wrote = false;
+#pragma clang loop unroll(full)
             for (int q = 0; q < RPT; q++)
                 if (rq[q] == wr && wr < n) {
+                    st_relaxed(sl + 0, dbits(wv));
+                    st_relaxed(sl + 1, (unsigned long long)(unsigned)wr);
                     st_relaxed(sl + 2, (unsigned long long)(unsigned)lorig[q] |
                                            ((unsigned long long)(unsigned)borig[q] << 32));
-#pragma unroll
+#pragma clang loop unroll(full)
                     for (int k = 0; k < W; k++) st_relaxed(sl + 3 + k, dbits((double)row[q][k]));
+                    wrote = true;
                 }
-            // the current row j lives in workgroup 0, thread jl, layer 0
+            if (wr >= n && t == 0) {
+                st_relaxed(sl + 0, dbits(-1.0));
+                st_relaxed(sl + 1, (unsigned long long)(unsigned)INT_MAX);
+                wrote = true;
+            }
             if (blockIdx.x == 0 && t == jl) {
                 unsigned long long *cs = slots + ((long)par * (G + 1) + G) * SLOT_WORDS;
                 st_relaxed(cs + 2, (unsigned long long)(unsigned)lorig[0] |
                                        ((unsigned long long)(unsigned)borig[0] << 32));
-#pragma unroll
+#pragma clang loop unroll(full)
                 for (int k = 0; k < W; k++) st_relaxed(cs + 3 + k, dbits((double)row[0][k]));
+                wrote = true;
             }
+            if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         }
         grid_barrier(a.bar, a.bar_base + (unsigned long long)(jl + 1) * G, a.err);
 
-        // ---- global winner over the G workgroup candidates (same in every WG)
+        // ---- one round trip: every slot (headers and rows) into LDS
+        constexpr int SW = W + 3;
+        for (int e = t; e < (G + 1) * SW; e += NTL) {
+            const int g2 = e / SW, k = e - g2 * SW;
+            xch[e] = ld_relaxed(slots + ((long)par * (G + 1) + g2) * SLOT_WORDS + k);
+        }
+        __syncthreads();
+        // ---- global winner over the G candidates (same in every workgroup)
         bv = -1.0;
         br = INT_MAX;
         int bg = -1;
         for (int g2 = t; g2 < G; g2 += NTL) {
-            unsigned long long *sl = slots + ((long)par * (G + 1) + g2) * SLOT_WORDS;
-            const double v = bitsd(ld_relaxed(sl + 0));
-            const int r = (int)(unsigned)ld_relaxed(sl + 1);
+            const double v = bitsd(xch[g2 * SW + 0]);
+            const int r = (int)(unsigned)xch[g2 * SW + 1];
             if (better(v, r, bv, br)) { bv = v; br = r; bg = g2; }
         }
 #pragma unroll
@@ -270,49 +291,36 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
             for (int w2 = 1; w2 < NTL / 64; w2++)
                 if (better(red_v[w2], red_r[w2], v, r)) { v = red_v[w2]; r = red_r[w2]; g2 = red_g[w2]; }
             s_win[1] = r;
-            s_win[2] = g2;
+            s_win[2] = g2 < 0 ? 0 : g2;
         }
         __syncthreads();
         const int p = s_win[1];
-        const int gw = s_win[2] < 0 ? 0 : s_win[2];
-        {
-            unsigned long long *ps = slots + ((long)par * (G + 1) + gw) * SLOT_WORDS;
-            unsigned long long *cs = slots + ((long)par * (G + 1) + G) * SLOT_WORDS;
-            if (t < W) piv_row[t] = (T)bitsd(ld_relaxed(ps + 3 + t));
-            else if (t < 2 * W) cur_row[t - W] = (T)bitsd(ld_relaxed(cs + 3 + (t - W)));
-            else if (t == 2 * W) {
-                const unsigned long long h = ld_relaxed(ps + 2);
-                s_win[0] = (int)(unsigned)(h & 0xffffffffULL);
-                s_win[3] = (int)(unsigned)(h >> 32);
-            } else if (t == 2 * W + 1) {
-                const unsigned long long h = ld_relaxed(cs + 2);
-                s_win[4] = (int)(unsigned)(h & 0xffffffffULL);
-                s_win[5] = (int)(unsigned)(h >> 32);
-            }
-        }
-        __syncthreads();
-        const int p_lorig = s_win[0], p_borig = s_win[3];
-        const int c_lorig = s_win[4], c_borig = s_win[5];
-        const T d = piv_row[jl];
+        const unsigned long long *pslot = xch + s_win[2] * SW;
+        const unsigned long long *cslot = xch + G * SW;
+        const int p_lorig = (int)(unsigned)(pslot[2] & 0xffffffffULL), p_borig = (int)(unsigned)(pslot[2] >> 32);
+        const int c_lorig = (int)(unsigned)(cslot[2] & 0xffffffffULL), c_borig = (int)(unsigned)(cslot[2] >> 32);
+#define PIVROW(k) ((T)bitsd(pslot[3 + (k)]))
+#define CURROW(k) ((T)bitsd(cslot[3 + (k)]))
+        const T d = PIVROW(jl);
         const T s = T(1) / d;
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int q = 0; q < RPT; q++) {
             const int r = rq[q];
             if (r == j) {
-#pragma unroll
-                for (int k = 0; k < W; k++) row[q][k] = (k == jl) ? d : piv_row[k] * s;
+#pragma clang loop unroll(full)
+                for (int k = 0; k < W; k++) row[q][k] = (k == jl) ? d : PIVROW(k) * s;
                 lorig[q] = p_lorig;
                 borig[q] = p_borig;
             } else if (r == p) {
-#pragma unroll
-                for (int k = 0; k < W; k++) row[q][k] = cur_row[k];
+#pragma clang loop unroll(full)
+                for (int k = 0; k < W; k++) row[q][k] = CURROW(k);
                 lorig[q] = c_lorig;
                 borig[q] = c_borig;
             }
             if (r > j && r < n) {
                 const T l = row[q][jl];
-#pragma unroll
-                for (int k = jl + 1; k < W; k++) row[q][k] -= l * (piv_row[k] * s);
+#pragma clang loop unroll(full)
+                for (int k = jl + 1; k < W; k++) row[q][k] -= l * (PIVROW(k) * s);
             }
         }
         if (t == 0) {
@@ -323,13 +331,15 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
             }
         }
     }
+#undef PIVROW
+#undef CURROW
     __syncthreads();
 
     // ---- write back rows (coalesced through LDS), origins and row pairs
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int q = 0; q < RPT; q++) {
         const long base = (long)c + (long)blockIdx.x * NTL + (long)q * TT;
-#pragma unroll
+#pragma clang loop unroll(full)
         for (int k = 0; k < W; k++) stage[t * (W + 1) + k] = row[q][k];
         __syncthreads();
         for (int e = t; e < NTL * W; e += NTL) {
@@ -341,7 +351,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
         const int r = rq[q];
         if (r < n) {
             a.orig[r] = borig[q];
-#pragma unroll
+#pragma clang loop unroll(full)
             for (int k = 0; k < W; k++)
                 if (pivs[k] == r) {
                     a.pair_dst[W + k] = r;
@@ -354,26 +364,26 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
             a.pair_dst[t] = c + t;
             a.pair_src[t] = lorig[0];
             // leaf block of the inverse of the unit-lower D^-1 L11
-#pragma unroll
+#pragma clang loop unroll(full)
             for (int k = 0; k < W; k++) stage[t * (W + 1) + k] = row[0][k];
         }
         __syncthreads();
         if (t < W) {
             T x[W];
-#pragma unroll
+#pragma clang loop unroll(full)
             for (int r = 0; r < W; r++) {
                 if (r < t) x[r] = T(0);
                 else if (r == t) x[r] = T(1);
                 else {
                     T acc = T(0);
-#pragma unroll
+#pragma clang loop unroll(full)
                     for (int k = 0; k < r; k++)
                         if (k >= t) acc += stage[r * (W + 1) + k] * x[k];
                     x[r] = -acc;
                 }
             }
             const long off = (long)(c - a.i);
-#pragma unroll
+#pragma clang loop unroll(full)
             for (int r = 0; r < W; r++) a.linv[(off + r) * a.ldl + off + t] = x[r];
         }
     }
@@ -652,7 +662,7 @@ template <> struct Prec<float> {
 
 template <typename T>
 static int gemm_call(lu2mi_ctx *ctx, int M, int N, int K, double alpha, const T *A, long lda,
-                     const T *B, long ldb, double beta, T *C, long ldc, int a_lower)
+                     const T *B, long ldb, double beta, T *C, long ldc, int a_lower, int tag = 0)
 {
     GemmDesc g;
     g.M = M; g.N = N; g.K = K; g.batch = 1;
@@ -661,6 +671,7 @@ static int gemm_call(lu2mi_ctx *ctx, int M, int N, int K, double alpha, const T 
     g.B = B; g.ldb = ldb; g.sB = 0;
     g.C = C; g.ldc = ldc; g.sC = 0;
     g.a_lower = a_lower;
+    g.tag = tag;
     return Prec<T>::gemm(g, ctx->stream);
 }
 
@@ -683,8 +694,8 @@ struct Factor {
         if (W == 16 && rpt > 4) rpt = 4;
         if (rpt == 3) rpt = 4;
         const int G = (R + NTL * rpt - 1) / (NTL * rpt);
-        if (G > 1024) {
-            lu2mi_set_error_msg("leaf: %d rows need %d workgroups (> 1024)", R, G);
+        if (G > LEAF_GMAX) {
+            lu2mi_set_error_msg("leaf: %d rows need %d workgroups (> %d)", R, G, LEAF_GMAX);
             return LU2MI_ERR_UNSUPPORTED;
         }
         LeafArgs<T> la;
@@ -780,7 +791,7 @@ struct Factor {
                 if (tm) LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used], ctx->stream));
                 LU2MI_TRY(gemm_call<T>(ctx, M, ncols, nb, -1.0, A + (long)(i + nb) * lda + i, lda,
                                        A + (long)i * lda + i + nb, lda, 1.0, A + (long)(i + nb) * lda + i + nb,
-                                       lda, 0));
+                                       lda, 0, 1));
                 if (tm) {
                     LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used + 1], ctx->stream));
                     kt.flops[kt.used] = 2.0 * M * (double)ncols * nb;
@@ -819,6 +830,8 @@ static int factor_impl(lu2mi_ctx *ctx, int n, T *dA, long lda, int nb, int pivmo
     f.compat = (pivmode == LU2MI_PIV_COMPAT);
     f.W = (ctx->leaf_w == 16) ? 16 : 32;
     if (f.W > nb / 2) f.W = 16;
+    // a W=32 leaf holds at most 2 rows per thread in registers over LEAF_GMAX workgroups
+    if (f.W == 32 && n > 2 * NTL * LEAF_GMAX) f.W = 16;
     f.linv = (T *)ctx->d_linv; f.tmp = (T *)ctx->d_tmp; f.wbuf = (T *)ctx->d_w; f.pairs = ctx->d_pairs;
     ctx->tgemm.used = 0;
     LU2MI_CHECK(hipMemsetAsync(ctx->d_err, 0, sizeof(int), ctx->stream));
