@@ -93,10 +93,11 @@ class DeviceBuffer:
         assert a.nbytes <= self.nbytes
         _check(_lib.lu2mi_memcpy_h2d(self.ctx.handle, self.ptr, a.ctypes.data, a.nbytes))
 
-    def download(self, dtype, count: int) -> np.ndarray:
+    def download(self, dtype, count: int, offset: int = 0) -> np.ndarray:
+        """count elements starting offset bytes into the buffer."""
         out = np.empty(count, dtype=dtype)
-        assert out.nbytes <= self.nbytes
-        _check(_lib.lu2mi_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        assert offset >= 0 and offset + out.nbytes <= self.nbytes
+        _check(_lib.lu2mi_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr + offset, out.nbytes))
         return out
 
     def copy_from(self, other: "DeviceBuffer", nbytes: int | None = None) -> None:
@@ -173,17 +174,18 @@ def solve(a_aug: np.ndarray, nb: int, pivmode: int = PIV_GEPP, ctx: Context | No
     try:
         a = np.ascontiguousarray(a_aug, dtype=np.float64)
         n, lda = a.shape
-        dA = ctx.alloc(a.nbytes)
-        dx = ctx.alloc(8 * n)
-        dp = ctx.alloc(4 * n)
-        dA.upload(a)
-        ctx.set_option(OPT_PIVMODE, pivmode)
-        ctx.lumcolumn_dev(n, dA, lda, nb, dp, dx)
-        x = dx.download(np.float64, n)
-        piv = dp.download(np.int32, n)
-        fac = dA.download(np.float64, n * lda).reshape(n, lda)
-        for b in (dA, dx, dp):
-            b.free()
+        bufs = [ctx.alloc(a.nbytes), ctx.alloc(8 * n), ctx.alloc(4 * n)]
+        dA, dx, dp = bufs
+        try:
+            dA.upload(a)
+            ctx.set_option(OPT_PIVMODE, pivmode)
+            ctx.lumcolumn_dev(n, dA, lda, nb, dp, dx)
+            x = dx.download(np.float64, n)
+            piv = dp.download(np.int32, n)
+            fac = dA.download(np.float64, n * lda).reshape(n, lda)
+        finally:
+            for b in bufs:
+                b.free()
         return x, piv, fac
     finally:
         if own:

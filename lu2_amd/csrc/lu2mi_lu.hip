@@ -7,7 +7,7 @@
 //
 //  per nb-block i:
 //    panel(i, nb)                       recursive halving down to a W-column leaf
-//      leaf   : one cooperative kernel factors W columns of rows [c, n) held in
+//      leaf   : one persistent kernel factors W columns of rows [c, n) held in
 //               registers; one grid barrier per column exchanges the
 //               per-workgroup pivot candidates (wavefront argmax, lowest row
 //               on ties) together with the candidate rows, so the winning row
@@ -29,7 +29,6 @@
 // in lu2.c:28), b in column n.  Row-major rows are contiguous, so row swaps
 // and scalings are coalesced along rows.
 #include "lu2mi_internal.h"
-#include <hip/hip_cooperative_groups.h>
 #include <limits.h>
 #include <stdarg.h>
 #include <stdlib.h>
@@ -82,7 +81,8 @@ __global__ __launch_bounds__(256) void randomsetmat_kernel(double *A, long lda, 
 }
 
 // ---------------------------------------------------------------------------
-// Leaf factorization: cooperative kernel over rows [c, n), W columns.
+// Leaf factorization: persistent kernel over rows [c, n), W columns; all
+// workgroups are co-resident and meet at a grid barrier once per column.
 namespace {
 constexpr int SLOT_WORDS = 40;  // 3 header words + up to 32 row values, padded
 constexpr int NTL = 256;        // threads per workgroup in the leaf kernel
@@ -290,8 +290,11 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
             int r = red_r[0], g2 = red_g[0];
             for (int w2 = 1; w2 < NTL / 64; w2++)
                 if (better(red_v[w2], red_r[w2], v, r)) { v = red_v[w2]; r = red_r[w2]; g2 = red_g[w2]; }
-            s_win[1] = r;
-            s_win[2] = g2 < 0 ? 0 : g2;
+            // no comparable candidate (every |x| is NaN): pivot on the current
+            // row, as BLAS idamax returns the first index for NaN input
+            const bool none = (r < j || r >= n || g2 < 0);
+            s_win[1] = none ? j : r;
+            s_win[2] = none ? G : g2;
         }
         __syncthreads();
         const int p = s_win[1];
@@ -710,7 +713,20 @@ struct Factor {
         if (W == 32) fn = (rpt == 1) ? (const void *)leaf_kernel<T, 32, 1> : (const void *)leaf_kernel<T, 32, 2>;
         else fn = (rpt == 1) ? (const void *)leaf_kernel<T, 16, 1>
                  : (rpt == 2) ? (const void *)leaf_kernel<T, 16, 2> : (const void *)leaf_kernel<T, 16, 4>;
-        LU2MI_CHECK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(NTL), args, 0, ctx->stream));
+        // All G workgroups must be resident together (they meet at the grid
+        // barrier).  An ordinary launch on an otherwise idle device places
+        // them all when occupancy allows it; the runtime's cooperative launch
+        // is not used because it holds a per-device resource that a second
+        // process on the same GPU cannot get.  The barrier's bounded spin
+        // turns any shortfall into LU2MI_ERR_DEVICE rather than a hang.
+        int per_cu = 0;
+        LU2MI_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTL, 0));
+        if ((long)per_cu * ctx->num_cu < G) {
+            lu2mi_set_error_msg("leaf: %d workgroups cannot be co-resident (%d per CU x %d CUs)", G, per_cu,
+                                ctx->num_cu);
+            return LU2MI_ERR_UNSUPPORTED;
+        }
+        LU2MI_CHECK(hipLaunchKernel(fn, dim3(G), dim3(NTL), args, 0, ctx->stream));
         ctx->bar_count += (unsigned long long)W * G;
         // apply the leaf's swaps to the other panel columns; L part (columns
         // left of the leaf) of the leaf's diagonal rows scaled by 1/d

@@ -60,6 +60,19 @@ void lu2o_randomsetmat(int n, int seed, double *a, long lda)
     }
 }
 
+/* Rows [r0, r0+nrows) of the lu2.c:70-85 matrix, by running the sequence
+ * sequentially past the first r0*(n+1) draws. */
+void lu2o_randomrows(int n, int seed, long r0, int nrows, double *a, long lda)
+{
+    unsigned long long st;
+    lu2o_srand48(seed, &st);
+    for (long e = 0; e < r0 * (long)(n + 1); e++) (void)lu2o_drand48(&st);
+    for (long r = 0; r < nrows; r++) {
+        for (long c = 0; c <= n; c++) A_(r, c) = lu2o_drand48(&st) - 0.5;
+        for (long c = n + 1; c < lda; c++) A_(r, c) = 0.0;
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Row-major GEMM, C = alpha*A*B + beta*C.  Stands in for cblas_dgemm as
  * called through mydgemm (lu2lib.c:2079-2135) and lu2tlib.c:1026.  Register

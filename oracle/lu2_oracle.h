@@ -33,6 +33,8 @@ void   lu2o_srand48(long seed, unsigned long long *state);
 double lu2o_drand48(unsigned long long *state);
 /* lu2.c:70-85 randomsetmat: a[i][j]=drand48()-0.5 row by row, b=a[i][n] last */
 void   lu2o_randomsetmat(int n, int seed, double *a, long lda);
+/* rows [r0, r0+nrows) of the same matrix, sequence run from the start */
+void   lu2o_randomrows(int n, int seed, long r0, int nrows, double *a, long lda);
 
 /* Row-major C = alpha*A*B + beta*C (lu2lib.c:2079 mydgemm semantics). */
 void   lu2o_dgemm(int m, int n, int k, double alpha, const double *a, long lda,

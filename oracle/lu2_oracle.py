@@ -35,6 +35,7 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(LIB)
         c_int, c_long, c_double = ctypes.c_int, ctypes.c_long, ctypes.c_double
         L.lu2o_randomsetmat.argtypes = [c_int, c_int, _dp, c_long]
+        L.lu2o_randomrows.argtypes = [c_int, c_int, c_long, c_int, _dp, c_long]
         L.lu2o_dgemm.argtypes = [c_int, c_int, c_int, c_double, _dp, c_long, _dp, c_long, c_double,
                                  _dp, c_long]
         L.lu2o_lumcolumn.argtypes = [c_int, _dp, c_long, c_int, c_int, _ip, _dp]
@@ -57,6 +58,13 @@ def randomsetmat(n: int, seed: int, lda: int | None = None) -> np.ndarray:
     lda = n + 16 if lda is None else lda
     a = np.zeros((n, lda))
     lib().lu2o_randomsetmat(n, seed, a, lda)
+    return a
+
+
+def randomrows(n: int, seed: int, r0: int, nrows: int, lda: int) -> np.ndarray:
+    """Rows [r0, r0+nrows) of the lu2.c:70-85 matrix."""
+    a = np.zeros((nrows, lda))
+    lib().lu2o_randomrows(n, seed, r0, nrows, a, lda)
     return a
 
 

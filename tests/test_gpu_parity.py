@@ -1,0 +1,218 @@
+"""GPU parity: the HIP path against the CPU oracle and the golden fixtures.
+
+Bars (north star): pivot sequence identical to the reference LU; x within
+1e-10 relative to ||x|| (gepp -- lu2's own COMPAT rule is numerically unstable,
+two correct implementations of it differ by up to ~1e-6 at n=2048, so COMPAT
+is held to identical pivots and 1e-5); HPL-2 scaled residual < 16 (gepp).
+"""
+import numpy as np
+import pytest
+
+import lu2_amd as L
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_solve(ctx, a0, nb, mode):
+    return L.solve(a0, nb, mode, ctx)
+
+
+# ---------------------------------------------------------------- generator
+@pytest.mark.parametrize("n,seed,lda", [(300, 1, 316), (1000, 12345, 1017), (64, 7, 80)])
+def test_randomsetmat_bit_exact(ctx, oracle, n, seed, lda):
+    dA = ctx.alloc(8 * n * lda)
+    dA.upload(np.full(n * lda, np.nan))
+    ctx.randomsetmat(n, seed, dA, lda)
+    got = dA.download(np.float64, n * lda).reshape(n, lda)
+    dA.free()
+    assert np.array_equal(got, oracle.randomsetmat(n, seed, lda))
+
+
+def test_randomsetmat_far_rows(ctx, oracle):
+    """Jump-ahead at full size: rows near the start and deep into an n=32768
+    matrix against the sequence run sequentially on the CPU."""
+    n, lda = 32768, 32784
+    dA = ctx.alloc(8 * n * lda)
+    try:
+        ctx.randomsetmat(n, 1, dA, lda)
+        head = dA.download(np.float64, 4 * lda).reshape(4, lda)
+        deep = dA.download(np.float64, 4 * lda, offset=8 * 20000 * lda).reshape(4, lda)
+    finally:
+        dA.free()
+    assert np.array_equal(head, oracle.randomrows(n, 1, 0, 4, lda))
+    assert np.array_equal(deep, oracle.randomrows(n, 1, 20000, 4, lda))
+
+
+# --------------------------------------------------------------------- GEMM
+def test_gemm_known_answer_identity(ctx):
+    """gdrdgemm.c:17-58: I(2048^2) * B == B exactly."""
+    N = 2048
+    ia = np.eye(N)
+    b = (np.arange(N)[:, None] * N + np.arange(N)[None, :]).astype(np.float64)
+    dA, dB, dC = ctx.alloc(ia.nbytes), ctx.alloc(b.nbytes), ctx.alloc(b.nbytes)
+    dA.upload(ia)
+    dB.upload(b)
+    dC.upload(np.zeros_like(b))
+    ctx.dgemm_dev(N, N, N, 1.0, dA, N, dB, N, 0.0, dC, N)
+    c = dC.download(np.float64, N * N).reshape(N, N)
+    for d in (dA, dB, dC):
+        d.free()
+    assert np.array_equal(c, b)
+
+
+@pytest.mark.parametrize("m,n,k,alpha,beta", [
+    (128, 128, 16, -1.0, 1.0), (300, 257, 100, -1.0, 1.0), (1000, 999, 512, -1.0, 1.0),
+    (64, 1, 512, 1.0, 0.0), (513, 130, 33, 0.75, -0.5), (1, 700, 3, 2.0, 1.0), (256, 256, 512, 1.0, 0.0)])
+def test_gemm_error_bound(ctx, m, n, k, alpha, beta):
+    rng = np.random.default_rng(m * 7 + n)
+    a, b, c = rng.standard_normal((m, k)), rng.standard_normal((k, n)), rng.standard_normal((m, n))
+    dA, dB, dC = ctx.alloc(a.nbytes), ctx.alloc(b.nbytes), ctx.alloc(c.nbytes)
+    dA.upload(a)
+    dB.upload(b)
+    dC.upload(c)
+    ctx.dgemm_dev(m, n, k, alpha, dA, k, dB, n, beta, dC, n)
+    got = dC.download(np.float64, m * n).reshape(m, n)
+    for d in (dA, dB, dC):
+        d.free()
+    ref = alpha * (a @ b) + beta * c
+    u = 2.0 ** -53
+    gamma = (k + 2) * u / (1 - (k + 2) * u)
+    bound = 2 * gamma * (abs(alpha) * (np.abs(a) @ np.abs(b)) + abs(beta) * np.abs(c))
+    assert np.all(np.abs(got - ref) <= bound + 1e-300)
+
+
+def test_gemm_strided_submatrix(ctx):
+    """Leading dimensions larger than the operand widths (lu2's RDIM = n+16)."""
+    rng = np.random.default_rng(3)
+    big = rng.standard_normal((600, 616))
+    m, n, k = 300, 280, 256
+    dX = ctx.alloc(big.nbytes)
+    dX.upload(big)
+    # C = rows 300.., cols 300..; A = rows 300.., cols 0..; B = rows 0.., cols 300..
+    ld = 616
+    offA, offB, offC = 8 * (300 * ld), 8 * (300), 8 * (300 * ld + 300)
+    ctx.dgemm_dev(m, n, k, -1.0, dX, ld, dX, ld, 1.0, dX, ld, offs=(offA, offB, offC))
+    got = dX.download(np.float64, 600 * ld).reshape(600, ld)
+    dX.free()
+    want = big.copy()
+    want[300:600, 300:580] -= big[300:600, 0:256] @ big[0:256, 300:580]
+    assert np.abs(got - want).max() < 1e-12 * 256 * 4
+    assert np.array_equal(got[:300], big[:300])
+
+
+# ----------------------------------------------------------------------- LU
+CASES = [(256, 64, 1), (512, 64, 12345), (1024, 128, 1), (1024, 256, 12345),
+         (2048, 256, 1), (2048, 512, 12345), (2048, 128, 7), (512, 32, 3)]
+
+
+@pytest.mark.parametrize("n,nb,seed", CASES)
+def test_lu_gepp_parity(ctx, oracle, n, nb, seed):
+    a0 = oracle.randomsetmat(n, seed)
+    x, piv, fac = gpu_solve(ctx, a0, nb, L.PIV_GEPP)
+    xo, po, fo = oracle.lumcolumn(a0, nb, oracle.PIV_GEPP)
+    assert np.array_equal(piv, po)
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+    assert oracle.residuals(a0, x)[6] < 16.0
+    # lu2's whole stored factor form (L/d in the diagonal blocks, unscaled
+    # L21, unit-upper U, transformed b in column n)
+    scale = np.abs(fo[:, : n + 1]).max()
+    assert np.abs(fac[:, : n + 1] - fo[:, : n + 1]).max() <= 1e-11 * scale
+
+
+@pytest.mark.parametrize("n,nb,seed", CASES)
+def test_lu_compat_parity(ctx, oracle, n, nb, seed):
+    a0 = oracle.randomsetmat(n, seed)
+    x, piv, _ = gpu_solve(ctx, a0, nb, L.PIV_COMPAT)
+    xo, po, _ = oracle.lumcolumn(a0, nb, oracle.PIV_COMPAT)
+    assert np.array_equal(piv, po)
+    assert np.abs(x - xo).max() <= 1e-5 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("n,seed", [(256, 1), (512, 12345), (1024, 1)])
+def test_lu_gepp_matches_lapack_fixture(ctx, oracle, n, seed):
+    g = golden(f"lapack_n{n}_s{seed}.npz")
+    a0 = oracle.randomsetmat(n, seed)
+    x, piv, _ = gpu_solve(ctx, a0, 128 if n >= 256 else 64, L.PIV_GEPP)
+    assert np.array_equal(piv, g["piv"])
+    assert np.abs(x - g["x"]).max() <= 1e-10 * np.abs(g["x"]).max()
+
+
+@pytest.mark.parametrize("n,seed", [(64, 1), (256, 12345)])
+def test_lu_compat_matches_rule_fixture(ctx, oracle, n, seed):
+    g = golden(f"compat_n{n}_s{seed}.npz")
+    a0 = oracle.randomsetmat(n, seed)
+    _, piv, _ = gpu_solve(ctx, a0, 32, L.PIV_COMPAT)
+    assert np.array_equal(piv, g["piv"])
+
+
+def test_lu_leaf16_and_wide_lda(ctx, oracle):
+    n, nb = 1024, 128
+    a0 = oracle.randomsetmat(n, 11, n + 40)
+    ctx.set_option(L.OPT_LEAF_W, 16)
+    try:
+        x, piv, _ = gpu_solve(ctx, a0, nb, L.PIV_GEPP)
+    finally:
+        ctx.set_option(L.OPT_LEAF_W, 32)
+    xo, po, _ = oracle.lumcolumn(a0, nb, oracle.PIV_GEPP)
+    assert np.array_equal(piv, po)
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
+def test_lu_single_block(ctx, oracle):
+    n = 256
+    a0 = oracle.randomsetmat(n, 2)
+    x, piv, _ = gpu_solve(ctx, a0, n, L.PIV_GEPP)
+    xo, po, _ = oracle.lumcolumn(a0, n, oracle.PIV_GEPP)
+    assert np.array_equal(piv, po)
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
+def test_lu_rejects_reference_limits(ctx, oracle):
+    a0 = oracle.randomsetmat(96, 1)
+    with pytest.raises(L.Lu2miError):
+        gpu_solve(ctx, a0, 48, L.PIV_GEPP)   # nb not a power of two
+    with pytest.raises(L.Lu2miError):
+        gpu_solve(ctx, a0, 64, L.PIV_GEPP)   # n % nb != 0
+
+
+def test_lu_singular_matrix_completes(ctx, oracle):
+    """A zero column gives a zero pivot: the call completes (no hang) and,
+    as in lu2, the solution is not finite."""
+    n = 256
+    a0 = oracle.randomsetmat(n, 4)
+    a0[:, 10] = 0.0
+    x, _, _ = gpu_solve(ctx, a0, 64, L.PIV_GEPP)
+    assert not np.all(np.isfinite(x))
+
+
+def test_lu_n4096_full_oracle(ctx, oracle):
+    n, nb = 4096, 512
+    a0 = oracle.randomsetmat(n, 1)
+    x, piv, _ = gpu_solve(ctx, a0, nb, L.PIV_GEPP)
+    xo, po, _ = oracle.lumcolumn(a0, nb, oracle.PIV_GEPP)
+    assert np.array_equal(piv, po)
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
+def test_lu_n32768_residual(ctx):
+    """Config 2 size (n=32768, nb=512): the HPL-2 scaled residual on the
+    device against the regenerated matrix, and the pivot sequence shape."""
+    n, nb = 32768, 512
+    lda = n + 16
+    dA0 = ctx.alloc(8 * n * lda)
+    dA = ctx.alloc(8 * n * lda)
+    dx = ctx.alloc(8 * n)
+    dp = ctx.alloc(4 * n)
+    try:
+        ctx.randomsetmat(n, 1, dA0, lda)
+        dA.copy_from(dA0)
+        ctx.set_option(L.OPT_PIVMODE, L.PIV_GEPP)
+        ctx.lumcolumn_dev(n, dA, lda, nb, dp, dx)
+        res = ctx.residual_dev(n, dA0, lda, dx)
+        piv = dp.download(np.int32, n)
+    finally:
+        for d in (dA0, dA, dx, dp):
+            d.free()
+    assert res[6] < 16.0
+    assert np.all(piv >= np.arange(n)) and np.all(piv < n)
