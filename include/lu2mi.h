@@ -89,6 +89,35 @@ int lu2mi_dgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, double alpha,
 int lu2mi_residual_dev(lu2mi_ctx *ctx, int n, const double *dA0, long lda,
                        const double *dx, double *out);
 
+/* ---- mixed precision ---------------------------------------------------- */
+#define LU2MI_PREC_FP64 0
+#define LU2MI_PREC_FP32 1
+
+typedef struct {
+    int    iterations;          /* refinement steps taken */
+    int    gmres_used;          /* 1 once classical refinement stalled and GMRES-IR took over */
+    int    gmres_inner;         /* total GMRES inner iterations */
+    double resid_hpl2_initial;  /* HPL-2 scaled residual of the low-precision solution */
+    double resid_hpl2;          /* ... after refinement (FP64, against the original matrix) */
+    double t_factor_ms;         /* conversion + low-precision factorization */
+    double t_refine_ms;         /* residuals, re-solves and updates */
+} lu2mi_ir_info;
+
+/* Solve lu2's augmented system (dA: FP64, b in column n, not modified) by an
+ * LU of a copy in precision prec (FP32: FP32 MFMA trailing updates with FP32
+ * accumulation; same pivot rule and factor form as lu2mi_lumcolumn_dev) and
+ * FP64 iterative refinement -- classical, switching to GMRES-IR preconditioned
+ * by the same factors when it stalls -- until the HPL-2 scaled residual is at
+ * most target or maxiter steps are done.  x is written to dx (FP64).  Returns
+ * LU2MI_ERR_NOCONV when the target is not reached (dx then holds the last
+ * iterate). */
+int lu2mi_solve_mixed(lu2mi_ctx *ctx, int n, const double *dA, long lda, int nb, int prec,
+                      double *dx, int maxiter, double target, lu2mi_ir_info *info);
+
+/* FP32 row-major GEMM on device pointers, C = alpha*A*B + beta*C (FP32 MFMA). */
+int lu2mi_sgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, float alpha, const float *dA, long lda,
+                    const float *dB, long ldb, float beta, float *dC, long ldc);
+
 /* Timing of the last lu2mi_lumcolumn_dev with LU2MI_OPT_TIMING on:
  * out[0]=total ms of trailing-update GEMMs (HIP events on the library
  * stream), out[1]=number of launches, out[2]=their algorithmic flops,

@@ -77,6 +77,15 @@ void matmul_for_small_nk(int n1, double *a, int n2, double *b, int n3, double *c
  * gone; a and awork are not touched. */
 void reset_gdr(int m, double *a, int nb, double *awork, int n);
 
+/* ---- lu2tlib.h (lu2tlib.h:2-18) ---------------------------------------- */
+
+/* lu2tlib.c:1130-1137 (recursion lu2tlib.c:1095-1128): factor the panel held
+ * column-major in a (a[c][r] = column c, row r, leading dimension n; rows
+ * [i, n) of m columns) with lu2's recursive panel and pivot rule, on the GPU;
+ * pv[0..m) receives the global pivot rows.  m must be 16 or 32 (lu2.c:1082
+ * calls it with 16). */
+void cm_column_decomposition_recursive(int n, double *a, int m, int *pv, int i);
+
 /* ---- lu2.c:1122 (the driver's LU + solve) ------------------------------ */
 
 /* lu2.c:1122-1144 lumcolumn(n, a, b, nb, awork, pv, recursive) on a host

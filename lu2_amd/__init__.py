@@ -23,6 +23,8 @@ LIB_PATH = os.path.join(_HERE, "liblu2mi.so")
 
 PIV_COMPAT = 0
 PIV_GEPP = 1
+PREC_FP64 = 0
+PREC_FP32 = 1
 OPT_PIVMODE = 1
 OPT_LEAF_W = 2
 OPT_TIMING = 3
@@ -32,6 +34,16 @@ _lib = None
 
 class Lu2miError(RuntimeError):
     pass
+
+
+class IRInfo(ctypes.Structure):
+    """lu2mi_ir_info (include/lu2mi.h)."""
+    _fields_ = [("iterations", ctypes.c_int), ("gmres_used", ctypes.c_int), ("gmres_inner", ctypes.c_int),
+                ("resid_hpl2_initial", ctypes.c_double), ("resid_hpl2", ctypes.c_double),
+                ("t_factor_ms", ctypes.c_double), ("t_refine_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def load_library(path: str | None = None) -> ctypes.CDLL:
@@ -63,6 +75,10 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
                                     c_double, vp, c_long]),
         "lu2mi_residual_dev": (c_int, [vp, c_int, vp, c_long, vp, vp]),
         "lu2mi_get_timing": (c_int, [vp, vp, c_int]),
+        "lu2mi_solve_mixed": (c_int, [vp, c_int, vp, c_long, c_int, c_int, vp, c_int, c_double,
+                                      ctypes.POINTER(IRInfo)]),
+        "lu2mi_sgemm_dev": (c_int, [vp, c_int, c_int, c_int, ctypes.c_float, vp, c_long, vp, c_long,
+                                    ctypes.c_float, vp, c_long]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -152,6 +168,21 @@ class Context:
     def dgemm_dev(self, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, offs=(0, 0, 0)) -> None:
         _check(_lib.lu2mi_dgemm_dev(self.handle, m, n, k, alpha, dA.ptr + offs[0], lda,
                                     dB.ptr + offs[1], ldb, beta, dC.ptr + offs[2], ldc))
+
+    def sgemm_dev(self, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc) -> None:
+        _check(_lib.lu2mi_sgemm_dev(self.handle, m, n, k, alpha, dA.ptr, lda, dB.ptr, ldb, beta, dC.ptr, ldc))
+
+    def solve_mixed(self, n, dA, lda, nb, dx, prec=PREC_FP32, maxiter=30, target=1.0,
+                    raise_on_noconv=True) -> dict:
+        """FP32 factorization + FP64 refinement; dA (FP64, b in column n) is not modified."""
+        info = IRInfo()
+        rc = _lib.lu2mi_solve_mixed(self.handle, n, dA.ptr, lda, nb, prec, dx.ptr, maxiter, target,
+                                    ctypes.byref(info))
+        if rc != 0 and not (rc == -7 and not raise_on_noconv):
+            _check(rc)
+        d = info.as_dict()
+        d["converged"] = rc == 0
+        return d
 
     def residual_dev(self, n, dA0, lda, dx) -> np.ndarray:
         out = np.zeros(10)

@@ -78,7 +78,8 @@ extern "C" int lu2mi_finalize(lu2mi_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->d_piv, ctx->d_orig, ctx->d_invd, ctx->d_invd_f, ctx->d_pairs, ctx->d_linv,
                     ctx->d_tmp, ctx->d_w, ctx->d_slots, ctx->d_bar, ctx->d_err, ctx->d_y,
-                    ctx->h_dev_a, ctx->h_dev_b, ctx->h_dev_c};
+                    ctx->h_dev_a, ctx->h_dev_b, ctx->h_dev_c, ctx->m_af, ctx->m_linv, ctx->m_pairs,
+                    ctx->m_vec};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     free_timer(ctx->tgemm);
@@ -421,6 +422,23 @@ extern "C" void reset_gdr(int m, double *a, int nb, double *awork, int n)
     gdr_check_and_restart((double(*)[LU2MI_NMAT])s, (double(*)[LU2MI_NMAT])(s + N * N),
                           (double(*)[LU2MI_NMAT])(s + 2 * N * N));
     free(s);
+}
+
+// lu2tlib.c:1130-1137 cm_column_decomposition_recursive(n, a, m, pv, i): the
+// column-major panel (awork of lu2.c:1072-1085: a[c][r], leading dimension n,
+// rows [i, n) of m columns) factored on the GPU in place with lu2's recursive
+// panel; pv[0..m) receives the global pivot rows.  Called by lu2.c:1082 with
+// m = 16.
+extern "C" void cm_column_decomposition_recursive(int n, double *a, int m, int *pv, int i)
+{
+    lu2mi_ctx *ctx = lu2mi_default_ctx();
+    const size_t bytes = sizeof(double) * (size_t)m * (size_t)n;
+    double *d = nullptr;
+    if (lu2mi_malloc(ctx, (void **)&d, bytes)) fatal("cm_column_decomposition_recursive allocation");
+    if (lu2mi_memcpy_h2d(ctx, d, a, bytes)) fatal("cm_column_decomposition_recursive copy-in");
+    if (lu2mi_cm_panel_f64(ctx, d, n, n, m, i, ctx->pivmode, pv)) fatal("cm_column_decomposition_recursive");
+    if (lu2mi_memcpy_d2h(ctx, a, d, bytes)) fatal("cm_column_decomposition_recursive copy-out");
+    lu2mi_free(ctx, d);
 }
 
 // lu2.c:1122-1144 lumcolumn on a host matrix a[n][n+16] (b in column n):

@@ -66,7 +66,7 @@ struct lu2mi_ctx {
     size_t w_bytes;
     void *d_slots;         // leaf kernel exchange slots
     size_t slots_bytes;
-    unsigned long long *d_bar;  // leaf grid-barrier counter (monotonic)
+    unsigned long long *d_bar;  // back-substitution frontier (solved block count)
     unsigned long long bar_count;
     int *d_err;            // device error flag
     double *d_y;           // back-substitution vector scratch
@@ -76,6 +76,11 @@ struct lu2mi_ctx {
     int timing_on;
     KernelTimer tgemm;
     double t_last_ms[8];
+    // mixed precision: FP32 working copy, kept per-block factors, refinement vectors
+    void *m_af; size_t m_af_bytes;
+    void *m_linv; size_t m_linv_bytes;
+    void *m_pairs; size_t m_pairs_bytes;
+    void *m_vec; size_t m_vec_bytes;
     // host staging buffers for the host-memory seams (mygdrdgemm, lumcolumn)
     void *h_dev_a; size_t h_dev_a_bytes;
     void *h_dev_b; size_t h_dev_b_bytes;
@@ -86,7 +91,12 @@ int lu2mi_ctx_reserve(lu2mi_ctx *ctx, int n, int nb, int elem_bytes);
 int lu2mi_factor_f64(lu2mi_ctx *ctx, int n, double *dA, long lda, int nb, int pivmode,
                      int *dpiv_out, double *dx);
 int lu2mi_factor_f32(lu2mi_ctx *ctx, int n, float *dA, long lda, int nb, int pivmode,
-                     int *dpiv_out, float *dx);
+                     int *dpiv_out, float *dx, float *keep_linv, int *keep_pairs);
+int lu2mi_factor_f64_keep(lu2mi_ctx *ctx, int n, double *dA, long lda, int nb, int pivmode, int *dpiv,
+                          double *dx, double *keep_linv, int *keep_pairs);
+int lu2mi_backsub_f32_f64(lu2mi_ctx *ctx, const float *A, long lda, int n, double *x);
+int lu2mi_cm_panel_f64(lu2mi_ctx *ctx, double *dcm, long ld, int n, int m, int i, int pivmode, int *hpv);
+int lu2mi_backsub_f64_f64(lu2mi_ctx *ctx, const double *A, long lda, int n, double *x);
 int lu2mi_dev_buf(void **p, size_t *cur, size_t need);
 
 // default context used by the lu2-shaped (reference-signature) entry points

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void randomsetmat_kernel(double *A, long lda, 
 // Leaf factorization: persistent kernel over rows [c, n), W columns; all
 // workgroups are co-resident and meet at a grid barrier once per column.
 namespace {
-constexpr int SLOT_WORDS = 40;  // 3 header words + up to 32 row values, padded
+constexpr int SLOT_WORDS = 40;  // 3 header words + up to 32 row values; the last word is the sequence number
 constexpr int NTL = 256;        // threads per workgroup in the leaf kernel
 constexpr int LEAF_GMAX = 128;  // most workgroups a leaf launch may use
 
@@ -98,30 +98,46 @@ __device__ __forceinline__ unsigned long long ld_relaxed(unsigned long long *p)
 }
 __device__ __forceinline__ unsigned long long dbits(double v) { return __double_as_longlong(v); }
 __device__ __forceinline__ double bitsd(unsigned long long v) { return __longlong_as_double(v); }
-
-// Monotonic-counter grid barrier.  Bounded spin: on time-out the device error
-// flag is raised and every later barrier falls through, so the kernel always
-// drains; the host turns the flag into LU2MI_ERR_DEVICE.
-__device__ __forceinline__ void grid_barrier(unsigned long long *bar, unsigned long long target,
-                                             int *err)
+// device-scope (coherent across XCDs) scalar loads and stores of a vector element
+__device__ __forceinline__ double ld_dev(const double *p)
 {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(bar, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned long spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(1);
-            if (((++spins) & 1023) == 0) {
-                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (spins > (1UL << 25)) {
-                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
+    return bitsd(__hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ float ld_dev(const float *p)
+{
+    return __uint_as_float(__hip_atomic_load((unsigned *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+// device-scope loads through a buffer resource: same coherence as ld_dev but
+// without a wait after every load, so independent loads overlap
+__device__ __forceinline__ double ld_buf(__amdgpu_buffer_rsrc_t r, int byte_off, double)
+{
+    const unsigned int lo = __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 16);
+    const unsigned int hi = __builtin_amdgcn_raw_buffer_load_b32(r, byte_off + 4, 0, 16);
+    return bitsd(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float ld_buf(__amdgpu_buffer_rsrc_t r, int byte_off, float)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 16));
+}
+__device__ __forceinline__ void st_dev(double *p, double v)
+{
+    __hip_atomic_store((unsigned long long *)p, dbits(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(float *p, float v)
+{
+    __hip_atomic_store((unsigned *)p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// rank-1 update of one element: x - l * (u * s), with u*s rounded first (the
+// stored U value) and one fused multiply-subtract, written out so that every
+// copy of a row computes bit-identical values
+__device__ __forceinline__ double pending_update(double x, double l, double u, double s)
+{
+    return fma(-l, u * s, x);
+}
+__device__ __forceinline__ float pending_update(float x, float l, float u, float s)
+{
+    return fmaf(-l, u * s, x);
 }
 
 __device__ __forceinline__ bool better(double v, int r, double bv, int br)
@@ -143,28 +159,41 @@ struct LeafArgs {
     T *linv;                      // nb x nb inverse of D^-1 L11 (this leaf's block)
     int ldl;
     unsigned long long *slots;    // 2 * (G+1) * SLOT_WORDS, G <= LEAF_GMAX
-    unsigned long long *bar;
-    unsigned long long bar_base;
+    unsigned long long seq_base;  // step sequence numbers seq_base+1 .. seq_base+W
     int *err;
 };
 
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Per column step j (one grid-wide exchange):
+//   gather   every workgroup waits for the step's sequence number in each
+//            slot and copies all slots (candidate value, row, origins and
+//            the W values of the candidate row, plus the current row j) into
+//            LDS with device-scope buffer loads;
+//   reduce   each wavefront reduces the G candidates (largest |x|, lowest
+//            row on ties) -- identical result everywhere, no extra barrier;
+//   swap     the owners of rows j and p exchange rows, row j scaled by 1/d;
+//   column j+1 updated first, its candidate reduced in the workgroup and
+//            published (lookahead), then the other columns updated while
+//            the exchange is in flight.
 template <typename T, int W, int RPT>
 __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
 {
+    constexpr int SW = W + 3;
     __shared__ T stage[NTL * (W + 1)];
-    // every workgroup's slot (header + row) plus the current-row slot
-    __shared__ unsigned long long xch[(LEAF_GMAX + 1) * (W + 3)];
+    __shared__ unsigned long long xch[2][(LEAF_GMAX + 1) * SW];
     __shared__ double red_v[NTL / 64];
-    __shared__ int red_r[NTL / 64], red_g[NTL / 64];
-    __shared__ int s_win[8];
+    __shared__ int red_r[NTL / 64];
     __shared__ int pivs[W];
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int G = a.G;
     const long TT = (long)G * NTL;
-    const int gt = blockIdx.x * NTL + t;
     const int n = a.n, c = a.c;
     T *__restrict__ A = a.A;
+    unsigned long long *const slots = a.slots;
+    const __amdgpu_buffer_rsrc_t srsrc =
+        __builtin_amdgcn_make_buffer_rsrc(slots, 0, (int)(2 * (G + 1) * SLOT_WORDS * 8), 0x00020000);
 
     T row[RPT][W];
     int lorig[RPT], borig[RPT], rq[RPT];
@@ -187,23 +216,26 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
         borig[q] = (rq[q] < n) ? ((c == a.i) ? rq[q] : a.orig[rq[q]]) : -1;
     }
 
-    unsigned long long *const slots = a.slots;
-#pragma clang loop unroll(full)
-    for (int jl = 0; jl < W; jl++) {
-        const int j = c + jl;
-        const int par = jl & 1;
-        const bool odd_rule = a.compat && (jl & 1);
-        // ---- local candidate: first max |x| over owned rows r >= j
+    // local candidate of column jc, workgroup reduction, publication of the
+    // slot (and of the current row by its owner in workgroup 0)
+    // pprev/sprev: the previous step's pivot slot and 1/d when that step's
+    // update of columns > jc is still pending (lookahead); the published
+    // values then carry it, computed exactly as the bulk update computes it
+    auto publish = [&](const int jc, const unsigned long long *pprev, const T sprev)
+                       __attribute__((always_inline)) {
+        const int j = c + jc;
+        const int par = jc & 1;
+        const unsigned long long seq = a.seq_base + (unsigned long long)jc + 1;
+        const bool odd_rule = a.compat && (jc & 1);
         double bv = -1.0;
         int br = INT_MAX;
 #pragma clang loop unroll(full)
         for (int q = 0; q < RPT; q++) {
             const int r = rq[q];
             if (r >= j && r < n) {
-                const T xa = row[q][jl];
-                const T xb = row[q][jl > 0 ? jl - 1 : 0];
-                const T x = odd_rule ? xb : xa;
-                const double v = fabs((double)x);
+                const T xa = row[q][jc];
+                const T xb = row[q][jc > 0 ? jc - 1 : 0];
+                const double v = fabs((double)(odd_rule ? xb : xa));
                 if (v > bv) { bv = v; br = r; }
             }
         }
@@ -215,65 +247,81 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
         }
         if (lane == 0) { red_v[wave] = bv; red_r[wave] = br; }
         __syncthreads();
-        if (t == 0) {
-            double v = red_v[0];
-            int r = red_r[0];
-            for (int w2 = 1; w2 < NTL / 64; w2++)
-                if (better(red_v[w2], red_r[w2], v, r)) { v = red_v[w2]; r = red_r[w2]; }
-            s_win[0] = r;
-            red_v[0] = v;
-        }
-        __syncthreads();
-        {
-            // publish: the owner of the workgroup's candidate row writes the
-            // slot (value, row, origins, W values); workgroup 0's thread jl
-            // writes the current row j.  Writers fence before the barrier.
-            const int wr = s_win[0];
-            const double wv = red_v[0];
-            unsigned long long *sl = slots + ((long)par * (G + 1) + blockIdx.x) * SLOT_WORDS;
-            bool wrote = false;
+        double wv = red_v[0];
+        int wr = red_r[0];
+#pragma unroll
+        for (int w2 = 1; w2 < NTL / 64; w2++)
+            if (better(red_v[w2], red_r[w2], wv, wr)) { wv = red_v[w2]; wr = red_r[w2]; }
+        unsigned long long *sl = slots + ((long)par * (G + 1) + blockIdx.x) * SLOT_WORDS;
+        auto value = [&](int q, int k) __attribute__((always_inline)) -> T {
+            if (pprev && k > jc) return pending_update(row[q][k], row[q][jc > 0 ? jc - 1 : 0], (T)bitsd(pprev[3 + k]), sprev);
+            return row[q][k];
+        };
 #pragma clang loop unroll(full)
-            for (int q = 0; q < RPT; q++)
-                if (rq[q] == wr && wr < n) {
-                    st_relaxed(sl + 0, dbits(wv));
-                    st_relaxed(sl + 1, (unsigned long long)(unsigned)wr);
-                    st_relaxed(sl + 2, (unsigned long long)(unsigned)lorig[q] |
-                                           ((unsigned long long)(unsigned)borig[q] << 32));
+        for (int q = 0; q < RPT; q++)
+            if (rq[q] == wr && wr < n) {
+                st_relaxed(sl + 0, dbits(wv));
+                st_relaxed(sl + 1, (unsigned long long)(unsigned)wr);
+                st_relaxed(sl + 2, (unsigned long long)(unsigned)lorig[q] |
+                                       ((unsigned long long)(unsigned)borig[q] << 32));
 #pragma clang loop unroll(full)
-                    for (int k = 0; k < W; k++) st_relaxed(sl + 3 + k, dbits((double)row[q][k]));
-                    wrote = true;
-                }
-            if (wr >= n && t == 0) {
-                st_relaxed(sl + 0, dbits(-1.0));
-                st_relaxed(sl + 1, (unsigned long long)(unsigned)INT_MAX);
-                wrote = true;
+                for (int k = 0; k < W; k++) st_relaxed(sl + 3 + k, dbits((double)value(q, k)));
+                __builtin_amdgcn_s_waitcnt(0);
+                st_relaxed(sl + SLOT_WORDS - 1, seq);
             }
-            if (blockIdx.x == 0 && t == jl) {
-                unsigned long long *cs = slots + ((long)par * (G + 1) + G) * SLOT_WORDS;
-                st_relaxed(cs + 2, (unsigned long long)(unsigned)lorig[0] |
-                                       ((unsigned long long)(unsigned)borig[0] << 32));
-#pragma clang loop unroll(full)
-                for (int k = 0; k < W; k++) st_relaxed(cs + 3 + k, dbits((double)row[0][k]));
-                wrote = true;
-            }
-            if (wrote) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (wr >= n && t == 0) {
+            st_relaxed(sl + 0, dbits(-1.0));
+            st_relaxed(sl + 1, (unsigned long long)(unsigned)INT_MAX);
+            __builtin_amdgcn_s_waitcnt(0);
+            st_relaxed(sl + SLOT_WORDS - 1, seq);
         }
-        grid_barrier(a.bar, a.bar_base + (unsigned long long)(jl + 1) * G, a.err);
+        if (blockIdx.x == 0 && t == jc) {
+            unsigned long long *cs = slots + ((long)par * (G + 1) + G) * SLOT_WORDS;
+            st_relaxed(cs + 2, (unsigned long long)(unsigned)lorig[0] |
+                                   ((unsigned long long)(unsigned)borig[0] << 32));
+#pragma clang loop unroll(full)
+            for (int k = 0; k < W; k++) st_relaxed(cs + 3 + k, dbits((double)value(0, k)));
+            __builtin_amdgcn_s_waitcnt(0);
+            st_relaxed(cs + SLOT_WORDS - 1, seq);
+        }
+    };
 
-        // ---- one round trip: every slot (headers and rows) into LDS
-        constexpr int SW = W + 3;
-        for (int e = t; e < (G + 1) * SW; e += NTL) {
-            const int g2 = e / SW, k = e - g2 * SW;
-            xch[e] = ld_relaxed(slots + ((long)par * (G + 1) + g2) * SLOT_WORDS + k);
+    publish(0, nullptr, T(0));
+#pragma clang loop unroll(full)
+    for (int jl = 0; jl < W; jl++) {
+        const int j = c + jl;
+        const int par = jl & 1;
+        const unsigned long long seq = a.seq_base + (unsigned long long)jl + 1;
+        unsigned long long *const xb = xch[par];
+        // ---- gather: thread g waits for slot g, then copies it to LDS
+        if (t <= G) {
+            unsigned long long *sl = slots + ((long)par * (G + 1) + t) * SLOT_WORDS;
+            unsigned long spins = 0;
+            while (ld_relaxed(sl + SLOT_WORDS - 1) != seq) {
+                __builtin_amdgcn_s_sleep(1);
+                if (((++spins) & 1023) == 0) {
+                    if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                    if (spins > (1UL << 25)) {
+                        __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const int boff = (int)(((long)par * (G + 1) + t) * SLOT_WORDS * 8);
+#pragma clang loop unroll(full)
+            for (int k = 0; k < SW; k++) {
+                const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(srsrc, boff + 8 * k, 0, 16);
+                xb[t * SW + k] = ((unsigned long long)v.y << 32) | v.x;
+            }
         }
         __syncthreads();
-        // ---- global winner over the G candidates (same in every workgroup)
-        bv = -1.0;
-        br = INT_MAX;
-        int bg = -1;
-        for (int g2 = t; g2 < G; g2 += NTL) {
-            const double v = bitsd(xch[g2 * SW + 0]);
-            const int r = (int)(unsigned)xch[g2 * SW + 1];
+        // ---- every wavefront reduces the G candidates (same result everywhere)
+        double bv = -1.0;
+        int br = INT_MAX, bg = -1;
+        for (int g2 = lane; g2 < G; g2 += 64) {
+            const double v = bitsd(xb[g2 * SW + 0]);
+            const int r = (int)(unsigned)xb[g2 * SW + 1];
             if (better(v, r, bv, br)) { bv = v; br = r; bg = g2; }
         }
 #pragma unroll
@@ -283,29 +331,19 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
             const int og = __shfl_xor(bg, off);
             if (better(ov, orr, bv, br)) { bv = ov; br = orr; bg = og; }
         }
-        if (lane == 0) { red_v[wave] = bv; red_r[wave] = br; red_g[wave] = bg; }
-        __syncthreads();
-        if (t == 0) {
-            double v = red_v[0];
-            int r = red_r[0], g2 = red_g[0];
-            for (int w2 = 1; w2 < NTL / 64; w2++)
-                if (better(red_v[w2], red_r[w2], v, r)) { v = red_v[w2]; r = red_r[w2]; g2 = red_g[w2]; }
-            // no comparable candidate (every |x| is NaN): pivot on the current
-            // row, as BLAS idamax returns the first index for NaN input
-            const bool none = (r < j || r >= n || g2 < 0);
-            s_win[1] = none ? j : r;
-            s_win[2] = none ? G : g2;
-        }
-        __syncthreads();
-        const int p = s_win[1];
-        const unsigned long long *pslot = xch + s_win[2] * SW;
-        const unsigned long long *cslot = xch + G * SW;
+        // no comparable candidate (every |x| is NaN): pivot on the current
+        // row, as BLAS idamax returns the first index for NaN input
+        const bool none = (br < j || br >= n || bg < 0);
+        const int p = none ? j : br;
+        const unsigned long long *pslot = xb + (none ? G : bg) * SW;
+        const unsigned long long *cslot = xb + G * SW;
         const int p_lorig = (int)(unsigned)(pslot[2] & 0xffffffffULL), p_borig = (int)(unsigned)(pslot[2] >> 32);
         const int c_lorig = (int)(unsigned)(cslot[2] & 0xffffffffULL), c_borig = (int)(unsigned)(cslot[2] >> 32);
 #define PIVROW(k) ((T)bitsd(pslot[3 + (k)]))
 #define CURROW(k) ((T)bitsd(cslot[3 + (k)]))
         const T d = PIVROW(jl);
         const T s = T(1) / d;
+        // ---- swap rows j and p; row j becomes the pivot row scaled by 1/d
 #pragma clang loop unroll(full)
         for (int q = 0; q < RPT; q++) {
             const int r = rq[q];
@@ -320,10 +358,27 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
                 lorig[q] = c_lorig;
                 borig[q] = c_borig;
             }
+        }
+        // ---- column j+1 first, then publish its candidate (lookahead)
+        if (jl + 1 < W) {
+#pragma clang loop unroll(full)
+            for (int q = 0; q < RPT; q++) {
+                const int r = rq[q];
+                if (r > j && r < n) {
+                    T &x1 = row[q][jl + 1 < W ? jl + 1 : 0];
+                    x1 = pending_update(x1, row[q][jl], PIVROW(jl + 1 < W ? jl + 1 : 0), s);
+                }
+            }
+            publish(jl + 1 < W ? jl + 1 : 0, pslot, s);
+        }
+        // ---- the remaining columns while the exchange is in flight
+#pragma clang loop unroll(full)
+        for (int q = 0; q < RPT; q++) {
+            const int r = rq[q];
             if (r > j && r < n) {
                 const T l = row[q][jl];
 #pragma clang loop unroll(full)
-                for (int k = jl + 1; k < W; k++) row[q][k] -= l * (PIVROW(k) * s);
+                for (int k = jl + 2; k < W; k++) row[q][k] = pending_update(row[q][k], l, PIVROW(k), s);
             }
         }
         if (t == 0) {
@@ -333,9 +388,9 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
                 a.invd[j] = s;
             }
         }
-    }
 #undef PIVROW
 #undef CURROW
+    }
     __syncthreads();
 
     // ---- write back rows (coalesced through LDS), origins and row pairs
@@ -464,6 +519,16 @@ __global__ void scale_copy_kernel(const T *A, long lda, int r0, int c0, int h, i
     }
 }
 
+// column-major panel (m columns of ld rows, rows [r0, r0+R)) <-> row-major (R x ldp)
+__global__ void cm_to_rm_kernel(const double *cm, long ld, int r0, int R, int m, double *rm, long ldp, int back)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)R * m) return;
+    const int r = (int)(e / m), col = (int)(e % m);
+    if (back) ((double *)cm)[(long)col * ld + r0 + r] = rm[(long)r * ldp + col];
+    else rm[(long)r * ldp + col] = cm[(long)col * ld + r0 + r];
+}
+
 template <typename T>
 __global__ void fill_kernel(T *p, long count, T v)
 {
@@ -479,60 +544,95 @@ __global__ void copy_col_kernel(const T *A, long lda, int n, int col, T *x)
 }
 
 // ---------------------------------------------------------------------------
-// Back substitution on the unit-upper U (lu2.c:110-116), blocks of BSB rows
-// from the bottom.  Step b: workgroup 0 first applies the solved block b+1 to
-// the rows of block b and then solves block b in LDS; the other workgroups
-// apply block b+1 to the rows above block b.
+// Back substitution on the unit-upper U (lu2.c:110-116) in one persistent
+// launch.  Rows are cut into 64-row blocks, solved bottom-up; block number
+// idx (0 = bottom) belongs to workgroup idx % G.  A workgroup accumulates
+// U[block rows][solved block] * x[solved block] for every block below it as
+// soon as the shared frontier (count of solved blocks) shows it solved, then
+// solves its 64x64 diagonal block in LDS with one wavefront, publishes x with
+// device-scope stores and advances the frontier.  x holds y on entry.
+// TF is the factor's element type, TX the vector's (FP64 for refinement).
 constexpr int BSB = 64;
 
-template <typename T>
-__global__ __launch_bounds__(256) void backsub_step_kernel(const T *A, long lda, int n, int b, T *x)
+template <typename TF, typename TX>
+__global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, long lda, int n, TX *x,
+                                                      unsigned long long *front, int *err)
 {
-    __shared__ T U[BSB][BSB + 1];
-    __shared__ T xs[BSB];
-    const int r0 = b * BSB;
-    const int r1 = min(n, r0 + BSB);
-    const int nrb = r1 - r0;
-    const int s0 = r1, s1 = min(n, r1 + BSB);   // previously solved block
-    const int t = threadIdx.x;
-    const int rows_per_wg = 64;
-    // rows handled by this workgroup
-    int rlo, rhi;
-    if (blockIdx.x == 0) { rlo = r0; rhi = r1; }
-    else { rlo = (blockIdx.x - 1) * rows_per_wg; rhi = min(r0, rlo + rows_per_wg); }
-    if (s1 > s0) {
-        if (t < s1 - s0) xs[t] = x[s0 + t];
-        __syncthreads();
-        // 4 threads per row, 16 columns each
-        const int rr = rlo + (t >> 2), part = t & 3;
-        T acc = T(0);
-        if (rr < rhi) {
-            const T *u = A + (long)rr * lda + s0;
-            for (int k = part * 16; k < min(s1 - s0, part * 16 + 16); k++) acc += u[k] * xs[k];
+    __shared__ TF Ud[BSB][BSB + 1];
+    __shared__ TX part[BSB];
+    __shared__ int s_front;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int nblk = (n + BSB - 1) / BSB;
+    const int G = gridDim.x;
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(x, 0, (int)(sizeof(TX) * n), 0x00020000);
+    for (int idx = blockIdx.x; idx < nblk; idx += G) {
+        const int b = nblk - 1 - idx;
+        const int r0 = b * BSB, r1 = min(n, r0 + BSB), nrb = r1 - r0;
+        for (int e = t; e < BSB * BSB; e += 256) {
+            const int rr = e / BSB, kk = e % BSB;
+            Ud[rr][kk] = (rr < nrb && kk < nrb) ? A[(long)(r0 + rr) * lda + r0 + kk] : TF(0);
         }
-        acc += __shfl_xor(acc, 1);
-        acc += __shfl_xor(acc, 2);
-        if (rr < rhi && part == 0) x[rr] -= acc;
-    }
-    if (blockIdx.x != 0) return;
-    __syncthreads();
-    // solve block b: x[r] -= sum_{k>r, k<r1} U[r][k] x[k], bottom up
-    for (int e = t; e < BSB * BSB; e += 256) {
-        const int rr = e / BSB, kk = e % BSB;
-        U[rr][kk] = (rr < nrb && kk < nrb) ? A[(long)(r0 + rr) * lda + r0 + kk] : T(0);
-    }
-    if (t < BSB) xs[t] = (t < nrb) ? x[r0 + t] : T(0);
-    __syncthreads();
-    if (t < 64) {
-        // one wave; lane t keeps x[r0+t] in a register, bottom-up
-        T xt = xs[t];
-        for (int rr = nrb - 1; rr >= 0; rr--) {
-            T v = (t > rr && t < nrb) ? U[rr][t] * xt : T(0);
+        // wave w accumulates rows r0 + 16w .. r0 + 16w + 15, lane = column in the solved block
+        TX acc[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[i] = TX(0);
+        int done = 0;
+        while (done < idx) {
+            if (t == 0) {
+                unsigned long spins = 0;
+                int f;
+                while ((f = (int)ld_relaxed(front)) <= done) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (((++spins) & 1023) == 0) {
+                        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { f = idx; break; }
+                        if (spins > (1UL << 25)) {
+                            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            f = idx;
+                            break;
+                        }
+                    }
+                }
+                s_front = f;
+            }
+            __syncthreads();
+            const int F = min(s_front, idx);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            for (int d = done; d < F; d++) {
+                const int c0 = (nblk - 1 - d) * BSB;
+                const int col = c0 + lane;
+                if (col < n) {
+                    const TX xv = ld_buf(xr, (int)(sizeof(TX) * col), TX(0));
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        const int r = r0 + w * 16 + i;
+                        if (r < r1) acc[i] += (TX)A[(long)r * lda + col] * xv;
+                    }
+                }
+            }
+            done = F;
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            TX v = acc[i];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (t == rr) xt -= v;
+            if (lane == 0) part[w * 16 + i] = v;
         }
-        if (t < nrb) x[r0 + t] = xt;
+        __syncthreads();
+        if (t < 64) {
+            TX xt = (t < nrb) ? x[r0 + t] - part[t] : TX(0);
+            for (int rr = nrb - 1; rr >= 0; rr--) {
+                TX v = (t > rr && t < nrb) ? (TX)Ud[rr][t] * xt : TX(0);
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (t == rr) xt -= v;
+            }
+            if (t < nrb) st_dev(x + r0 + t, xt);
+            __builtin_amdgcn_s_waitcnt(0);
+            if (t == 0) st_relaxed(front, (unsigned long long)(idx + 1));
+        }
+        __syncthreads();
     }
 }
 
@@ -678,6 +778,23 @@ static int gemm_call(lu2mi_ctx *ctx, int M, int N, int K, double alpha, const T 
     return Prec<T>::gemm(g, ctx->stream);
 }
 
+// x <- U^-1 x with the unit-upper U stored in A (one persistent launch)
+template <typename TF, typename TX>
+static int backsub(lu2mi_ctx *ctx, const TF *A, long lda, int n, TX *x)
+{
+    const int nblk = (n + BSB - 1) / BSB;
+    int per_cu = 0;
+    LU2MI_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)backsub_kernel<TF, TX>, 256, 0));
+    int G = ctx->num_cu < nblk ? ctx->num_cu : nblk;
+    if (G > per_cu * ctx->num_cu) G = per_cu * ctx->num_cu;
+    if (G < 1) G = 1;
+    LU2MI_CHECK(hipMemsetAsync(ctx->d_bar, 0, sizeof(unsigned long long), ctx->stream));
+    hipLaunchKernelGGL((backsub_kernel<TF, TX>), dim3(G), dim3(256), 0, ctx->stream, A, lda, n, x, ctx->d_bar,
+                       ctx->d_err);
+    LU2MI_CHECK(hipGetLastError());
+    return LU2MI_OK;
+}
+
 template <typename T>
 struct Factor {
     lu2mi_ctx *ctx;
@@ -686,6 +803,10 @@ struct Factor {
     int n, nb, compat, W;
     T *linv, *tmp, *wbuf;
     int *pairs;
+    // when set, every block keeps its inverse diagonal block and its row
+    // pairs here (for later solves with the stored factors)
+    T *linv_all = nullptr;
+    int *pairs_all = nullptr;
 
     int leaf(int i, int c)
     {
@@ -707,7 +828,7 @@ struct Factor {
         la.pair_dst = pairs; la.pair_src = pairs + 2 * W;
         la.linv = linv; la.ldl = nb;
         la.slots = (unsigned long long *)ctx->d_slots;
-        la.bar = ctx->d_bar; la.bar_base = ctx->bar_count; la.err = ctx->d_err;
+        la.seq_base = ctx->bar_count; la.err = ctx->d_err;
         void *args[] = {&la};
         const void *fn = nullptr;
         if (W == 32) fn = (rpt == 1) ? (const void *)leaf_kernel<T, 32, 1> : (const void *)leaf_kernel<T, 32, 2>;
@@ -727,7 +848,7 @@ struct Factor {
             return LU2MI_ERR_UNSUPPORTED;
         }
         LU2MI_CHECK(hipLaunchKernel(fn, dim3(G), dim3(NTL), args, 0, ctx->stream));
-        ctx->bar_count += (unsigned long long)W * G;
+        ctx->bar_count += (unsigned long long)W;
         // apply the leaf's swaps to the other panel columns; L part (columns
         // left of the leaf) of the leaf's diagonal rows scaled by 1/d
         return rowperm(pairs, pairs + 2 * W, 2 * W, i, i + nb, c, c + W, c, c, c + W, nullptr, 0, 0, 0);
@@ -737,9 +858,19 @@ struct Factor {
                 int d1, T *Wb, long ldw, int w0, int w1)
     {
         if (c1 <= c0) return LU2MI_OK;
-        int sw = (int)(16384 / (np * (int)sizeof(T)));
+        static bool lds_attr = false;
+        if (!lds_attr) {
+            LU2MI_CHECK(hipFuncSetAttribute((const void *)rowperm_kernel<T>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
+            lds_attr = true;
+        }
+        // strip width: every source row of the strip fits in 128 KB of LDS
+        int sw = (int)(131072 / (np * (int)sizeof(T)));
         if (sw > 128) sw = 128;
-        if (sw < 8) sw = 8;
+        if (sw < 1) {
+            lu2mi_set_error_msg("rowperm: %d row pairs do not fit in LDS", np);
+            return LU2MI_ERR_UNSUPPORTED;
+        }
         PermArgs<T> p;
         p.A = A; p.lda = lda; p.dst = dst; p.src = src; p.np = np;
         p.c0 = c0; p.c1 = c1; p.k0 = k0; p.k1 = k1; p.sce = sce; p.d0 = d0; p.d1 = d1;
@@ -784,6 +915,10 @@ struct Factor {
     int run(int *dpiv_out, T *dx)
     {
         for (int i = 0; i < n; i += nb) {
+            if (linv_all) {
+                linv = linv_all + (long)(i / nb) * nb * nb;
+                pairs = pairs_all + (long)(i / nb) * 4 * nb;
+            }
             hipLaunchKernelGGL(fill_kernel<T>, dim3(256), dim3(256), 0, ctx->stream, linv, (long)nb * nb, T(0));
             LU2MI_CHECK(hipGetLastError());
             LU2MI_TRY(panel(i, i, nb));
@@ -818,13 +953,7 @@ struct Factor {
         // x = column n, then unit-upper back substitution
         hipLaunchKernelGGL(copy_col_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A, lda, n, n, dx);
         LU2MI_CHECK(hipGetLastError());
-        const int nblk = (n + BSB - 1) / BSB;
-        for (int b = nblk - 1; b >= 0; b--) {
-            const int above = b * BSB;
-            const int grid = 1 + (above + 63) / 64;
-            hipLaunchKernelGGL(backsub_step_kernel<T>, dim3(grid), dim3(256), 0, ctx->stream, A, lda, n, b, dx);
-            LU2MI_CHECK(hipGetLastError());
-        }
+        LU2MI_TRY(backsub(ctx, A, lda, n, dx));
         if (dpiv_out && dpiv_out != ctx->d_piv)
             LU2MI_CHECK(hipMemcpyAsync(dpiv_out, ctx->d_piv, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
         return LU2MI_OK;
@@ -832,10 +961,11 @@ struct Factor {
 };
 
 template <typename T>
-static int factor_impl(lu2mi_ctx *ctx, int n, T *dA, long lda, int nb, int pivmode, int *dpiv_out, T *dx)
+static int factor_impl(lu2mi_ctx *ctx, int n, T *dA, long lda, int nb, int pivmode, int *dpiv_out, T *dx,
+                       T *keep_linv = nullptr, int *keep_pairs = nullptr)
 {
-    if (!ctx || !dA || n <= 0 || lda < n + 1 || nb < 32 || (nb & (nb - 1)) || n % nb) {
-        lu2mi_set_error_msg("lumcolumn: need nb power of two >= 32, n %% nb == 0, lda >= n+1 "
+    if (!ctx || !dA || n <= 0 || lda < n + 1 || nb < 32 || nb > 4096 || (nb & (nb - 1)) || n % nb) {
+        lu2mi_set_error_msg("lumcolumn: need nb a power of two in [32, 4096], n %% nb == 0, lda >= n+1 "
                             "(n=%d nb=%d lda=%ld)", n, nb, lda);
         return LU2MI_ERR_ARG;
     }
@@ -849,9 +979,12 @@ static int factor_impl(lu2mi_ctx *ctx, int n, T *dA, long lda, int nb, int pivmo
     // a W=32 leaf holds at most 2 rows per thread in registers over LEAF_GMAX workgroups
     if (f.W == 32 && n > 2 * NTL * LEAF_GMAX) f.W = 16;
     f.linv = (T *)ctx->d_linv; f.tmp = (T *)ctx->d_tmp; f.wbuf = (T *)ctx->d_w; f.pairs = ctx->d_pairs;
+    f.linv_all = keep_linv;
+    f.pairs_all = keep_pairs;
     ctx->tgemm.used = 0;
     LU2MI_CHECK(hipMemsetAsync(ctx->d_err, 0, sizeof(int), ctx->stream));
-    LU2MI_CHECK(hipMemsetAsync(ctx->d_bar, 0, sizeof(unsigned long long), ctx->stream));
+    // slot sequence numbers restart with every factorization
+    LU2MI_CHECK(hipMemsetAsync(ctx->d_slots, 0, ctx->slots_bytes, ctx->stream));
     ctx->bar_count = 0;
     auto t0 = std::chrono::steady_clock::now();
     int rc = f.run(dpiv_out, x);
@@ -872,9 +1005,74 @@ int lu2mi_factor_f64(lu2mi_ctx *ctx, int n, double *dA, long lda, int nb, int pi
 {
     return factor_impl<double>(ctx, n, dA, lda, nb, pivmode, dpiv, dx);
 }
-int lu2mi_factor_f32(lu2mi_ctx *ctx, int n, float *dA, long lda, int nb, int pivmode, int *dpiv, float *dx)
+int lu2mi_factor_f32(lu2mi_ctx *ctx, int n, float *dA, long lda, int nb, int pivmode, int *dpiv, float *dx,
+                     float *keep_linv, int *keep_pairs)
 {
-    return factor_impl<float>(ctx, n, dA, lda, nb, pivmode, dpiv, dx);
+    return factor_impl<float>(ctx, n, dA, lda, nb, pivmode, dpiv, dx, keep_linv, keep_pairs);
+}
+int lu2mi_factor_f64_keep(lu2mi_ctx *ctx, int n, double *dA, long lda, int nb, int pivmode, int *dpiv,
+                          double *dx, double *keep_linv, int *keep_pairs)
+{
+    return factor_impl<double>(ctx, n, dA, lda, nb, pivmode, dpiv, dx, keep_linv, keep_pairs);
+}
+// lu2tlib.c:1130 cm_column_decomposition_recursive on a device copy: the
+// panel's rows [i, n) of m columns (column-major, leading dimension ld)
+// factored in place with the recursive panel; pv[k] = global pivot row.
+int lu2mi_cm_panel_f64(lu2mi_ctx *ctx, double *dcm, long ld, int n, int m, int i, int pivmode, int *hpv)
+{
+    if (m != 16 && m != 32) {
+        lu2mi_set_error_msg("cm panel: %d columns (16 or 32 supported)", m);
+        return LU2MI_ERR_UNSUPPORTED;
+    }
+    const int R = n - i;
+    if (R < m) {
+        lu2mi_set_error_msg("cm panel: %d rows for %d columns", R, m);
+        return LU2MI_ERR_ARG;
+    }
+    const long ldp = 2 * m;
+    LU2MI_TRY(lu2mi_ctx_reserve(ctx, R, 2 * m > 64 ? 2 * m : 64, sizeof(double)));
+    double *P = nullptr;
+    LU2MI_CHECK(hipMalloc(&P, sizeof(double) * (size_t)R * ldp));
+    const long tot = (long)R * m;
+    hipLaunchKernelGGL(cm_to_rm_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, dcm, ld, i,
+                       R, m, P, ldp, 0);
+    Factor<double> f;
+    f.ctx = ctx; f.A = P; f.lda = ldp; f.n = R; f.nb = m; f.compat = (pivmode == LU2MI_PIV_COMPAT); f.W = m;
+    f.linv = (double *)ctx->d_linv; f.tmp = (double *)ctx->d_tmp; f.wbuf = (double *)ctx->d_w;
+    f.pairs = ctx->d_pairs;
+    int rc = LU2MI_OK;
+    if (hipMemsetAsync(ctx->d_err, 0, sizeof(int), ctx->stream) != hipSuccess ||
+        hipMemsetAsync(ctx->d_slots, 0, ctx->slots_bytes, ctx->stream) != hipSuccess)
+        rc = LU2MI_ERR_HIP;
+    ctx->bar_count = 0;
+    if (rc == LU2MI_OK) rc = f.panel(0, 0, m);
+    if (rc == LU2MI_OK) {
+        hipLaunchKernelGGL(cm_to_rm_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, dcm, ld,
+                           i, R, m, P, ldp, 1);
+        int piv[32], herr = 0;
+        if (hipMemcpyAsync(piv, ctx->d_piv, sizeof(int) * m, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(&herr, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = LU2MI_ERR_HIP;
+        else if (herr)
+            rc = LU2MI_ERR_DEVICE;
+        else
+            for (int k = 0; k < m; k++) hpv[k] = i + piv[k];
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(P);
+    if (rc == LU2MI_ERR_HIP) lu2mi_set_error_msg("cm panel: HIP call failed");
+    if (rc == LU2MI_ERR_DEVICE) lu2mi_set_error_msg("cm panel: device wait timed out");
+    return rc;
+}
+
+int lu2mi_backsub_f32_f64(lu2mi_ctx *ctx, const float *A, long lda, int n, double *x)
+{
+    return backsub<float, double>(ctx, A, lda, n, x);
+}
+int lu2mi_backsub_f64_f64(lu2mi_ctx *ctx, const double *A, long lda, int n, double *x)
+{
+    return backsub<double, double>(ctx, A, lda, n, x);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,456 @@
+// lu2mi_mixed.hip -- mixed-precision solve: the LU of a lower-precision copy
+// of lu2's augmented matrix (same kernels as the FP64 path, FP32 MFMA for the
+// trailing updates with FP32 accumulation), then iterative refinement in
+// FP64 against the original matrix: r = b - A x, d = (LU)^-1 r with the
+// stored low-precision factors, x += d.  When classical refinement stalls,
+// GMRES preconditioned by the same factors takes over (GMRES-IR).
+//
+// The re-solve follows lu2's stored factor form (lu2.c:524-581): per block,
+// the block's row permutation and 1/d scaling, y = (D^-1 L11)^-1 r_block with
+// the block's kept inverse, r_below -= L21 y; then the unit-upper back
+// substitution (lu2.c:110-116).
+#include "lu2mi_internal.h"
+#include <math.h>
+#include <string.h>
+#include <chrono>
+#include <vector>
+
+namespace {
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+}  // namespace
+
+// FP64 augmented matrix (n x (n+1) used) -> FP32 copy with leading dimension ldf
+__global__ __launch_bounds__(256) void cvt_to_f32_kernel(const double *A, long lda, int n, float *F, long ldf)
+{
+    const long r = blockIdx.x;
+    const double *a = A + r * lda;
+    float *f = F + r * ldf;
+    for (long c = threadIdx.x; c < ldf; c += 256) f[c] = (c <= n) ? (float)a[c] : 0.f;
+}
+
+// One block of the forward solve: r[dst] = r[src] over the block's row
+// pairs (all reads first), rows of the block scaled by 1/d, then
+// r_block = Linv * r_block.  One workgroup.
+template <typename TF>
+__global__ __launch_bounds__(1024) void fwd_block_kernel(double *r, int i, int nb, const int *pairs, const TF *invd,
+                                                         const TF *linv)
+{
+    extern __shared__ double sh[];
+    double *tmp = sh;            // 2*nb gathered values
+    double *rb = sh + 2 * nb;    // nb scaled block values
+    const int t = threadIdx.x;
+    const int *dst = pairs, *src = pairs + 2 * nb;
+    for (int p = t; p < 2 * nb; p += blockDim.x) tmp[p] = r[src[p]];
+    __syncthreads();
+    for (int p = t; p < 2 * nb; p += blockDim.x) {
+        const int d = dst[p];
+        if (d < i || d >= i + nb) r[d] = tmp[p];   // rows outside the block
+        else rb[d - i] = tmp[p] * (double)invd[d];
+    }
+    __syncthreads();
+    for (int k = t; k < nb; k += blockDim.x) {
+        const TF *row = linv + (long)k * nb;
+        double s = 0.0;
+        for (int m = 0; m <= k; m++) s = fma((double)row[m], rb[m], s);
+        r[i + k] = s;
+    }
+}
+
+// r[j] -= sum_k A[j][i+k] * r[i+k] for rows j in [i+nb, n); one wavefront per row
+template <typename TF>
+__global__ __launch_bounds__(256) void l21_gemv_kernel(const TF *A, long lda, int n, int i, int nb, double *r)
+{
+    extern __shared__ double y[];
+    for (int k = threadIdx.x; k < nb; k += 256) y[k] = r[i + k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const long j = (long)i + nb + (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= n) return;
+    const TF *a = A + j * lda + i;
+    double s = 0.0;
+    for (int k = lane; k < nb; k += 64) s = fma((double)a[k], y[k], s);
+    s = wave_sum(s);
+    if (lane == 0) r[j] -= s;
+}
+
+// out = b - A v (mode 1, b = column n) or out = A v (mode 0); also |row| sums
+// of A when rowabs is given.  FP64, one wavefront per row.
+__global__ __launch_bounds__(256) void gemv_kernel(const double *A, long lda, int n, const double *v, double *out,
+                                                   int mode, double *rowabs)
+{
+    const int lane = threadIdx.x & 63;
+    const long j = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= n) return;
+    const double *a = A + j * lda;
+    double s = 0.0, sa = 0.0;
+    for (int c = lane; c < n; c += 64) {
+        s = fma(a[c], v[c], s);
+        if (rowabs) sa += fabs(a[c]);
+    }
+    s = wave_sum(s);
+    if (rowabs) sa = wave_sum(sa);
+    if (lane == 0) {
+        out[j] = mode ? a[n] - s : s;
+        if (rowabs) rowabs[j] = sa;
+    }
+}
+
+// per-workgroup partials: part[g*m + q] = sum over the workgroup's range of
+// x*V_q (dots, q < m), or max |x| (m == 0 -> part[g] = max|x|, part[G+g] = sum x^2)
+__global__ __launch_bounds__(256) void multidot_kernel(const double *x, const double *V, long ldv, int m, int n,
+                                                       double *part)
+{
+    __shared__ double sh[256];
+    const int t = threadIdx.x;
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long lo = (long)blockIdx.x * chunk, hi = min((long)n, lo + chunk);
+    if (m == 0) {
+        double mx = 0.0, ss = 0.0;
+        for (long e = lo + t; e < hi; e += 256) {
+            mx = fmax(mx, fabs(x[e]));
+            ss = fma(x[e], x[e], ss);
+        }
+        sh[t] = mx;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (t < st) sh[t] = fmax(sh[t], sh[t + st]);
+            __syncthreads();
+        }
+        if (t == 0) part[blockIdx.x] = sh[0];
+        __syncthreads();
+        sh[t] = ss;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (t < st) sh[t] += sh[t + st];
+            __syncthreads();
+        }
+        if (t == 0) part[gridDim.x + blockIdx.x] = sh[0];
+        return;
+    }
+    for (int q = 0; q < m; q++) {
+        const double *vq = V + (long)q * ldv;
+        double s = 0.0;
+        for (long e = lo + t; e < hi; e += 256) s = fma(x[e], vq[e], s);
+        sh[t] = s;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (t < st) sh[t] += sh[t + st];
+            __syncthreads();
+        }
+        if (t == 0) part[(long)blockIdx.x * m + q] = sh[0];
+        __syncthreads();
+    }
+}
+
+// y = alpha*y + sum_q c[q] * V_q   (c in device memory, m terms)
+__global__ __launch_bounds__(256) void combine_kernel(double *y, double alpha, const double *V, long ldv,
+                                                      const double *c, int m, int n)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    double s = alpha == 0.0 ? 0.0 : alpha * y[e];
+    for (int q = 0; q < m; q++) s = fma(c[q], V[(long)q * ldv + e], s);
+    y[e] = s;
+}
+
+// ===========================================================================
+namespace {
+constexpr int NPART = 256;
+
+struct Mixed {
+    lu2mi_ctx *ctx;
+    int n, nb;
+    const double *A;     // FP64 augmented matrix (b in column n), untouched
+    long lda;
+    float *F;            // FP32 factors
+    long ldf;
+    float *linv_all;
+    int *pairs_all;
+    double *vec;         // work vectors
+    double *part;        // reduction partials (host-readable after sync)
+    double *coef;        // coefficients for combine_kernel
+    double anorm = 0, bnorm = 0;
+    hipStream_t s() const { return ctx->stream; }
+
+    double *V(int q) const { return vec + (long)q * n; }
+
+    // r <- (LU)^-1 r with the FP32 factors, FP64 vector arithmetic
+    int lu_solve(double *r)
+    {
+        const size_t shf = sizeof(double) * 3 * nb;
+        for (int i = 0; i < n; i += nb) {
+            const int b = i / nb;
+            hipLaunchKernelGGL((fwd_block_kernel<float>), dim3(1), dim3(nb < 1024 ? nb : 1024), shf, s(), r, i, nb,
+                               pairs_all + (long)b * 4 * nb, (const float *)ctx->d_invd_f,
+                               linv_all + (long)b * nb * nb);
+            LU2MI_CHECK(hipGetLastError());
+            const int rows = n - i - nb;
+            if (rows > 0) {
+                hipLaunchKernelGGL((l21_gemv_kernel<float>), dim3((rows + 3) / 4), dim3(256), sizeof(double) * nb,
+                                   s(), (const float *)F, ldf, n, i, nb, r);
+                LU2MI_CHECK(hipGetLastError());
+            }
+        }
+        return lu2mi_backsub_f32_f64(ctx, F, ldf, n, r);
+    }
+
+    // r <- b - A x (FP64); returns ||r||_oo and ||x||_oo
+    int residual(const double *x, double *r, double *rn, double *xn)
+    {
+        hipLaunchKernelGGL(gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s(), A, lda, n, x, r, 1, (double *)nullptr);
+        LU2MI_CHECK(hipGetLastError());
+        double a[2];
+        LU2MI_TRY(norms(r, a));
+        *rn = a[0];
+        LU2MI_TRY(norms(x, a));
+        *xn = a[0];
+        return LU2MI_OK;
+    }
+
+    // a[0] = max|v|, a[1] = ||v||_2
+    int norms(const double *v, double *a)
+    {
+        hipLaunchKernelGGL(multidot_kernel, dim3(NPART), dim3(256), 0, s(), v, (const double *)nullptr, 0L, 0, n,
+                           part);
+        LU2MI_CHECK(hipGetLastError());
+        double h[2 * NPART];
+        LU2MI_CHECK(hipMemcpyAsync(h, part, sizeof h, hipMemcpyDeviceToHost, s()));
+        LU2MI_CHECK(hipStreamSynchronize(s()));
+        double mx = 0, ss = 0;
+        for (int g = 0; g < NPART; g++) {
+            mx = fmax(mx, h[g]);
+            ss += h[NPART + g];
+        }
+        a[0] = mx;
+        a[1] = sqrt(ss);
+        return LU2MI_OK;
+    }
+
+    // out[q] = <x, V_q>, q < m
+    int dots(const double *x, int m, double *out)
+    {
+        hipLaunchKernelGGL(multidot_kernel, dim3(NPART), dim3(256), 0, s(), x, vec, (long)n, m, n, part);
+        LU2MI_CHECK(hipGetLastError());
+        std::vector<double> h((size_t)NPART * m);
+        LU2MI_CHECK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s()));
+        LU2MI_CHECK(hipStreamSynchronize(s()));
+        for (int q = 0; q < m; q++) {
+            double acc = 0;
+            for (int g = 0; g < NPART; g++) acc += h[(size_t)g * m + q];
+            out[q] = acc;
+        }
+        return LU2MI_OK;
+    }
+
+    // y = alpha*y + sum c[q] V_q, V_q = base + q*n (base defaults to the Krylov vectors)
+    int combine(double *y, double alpha, const double *c, int m, const double *base = nullptr)
+    {
+        if (m > 0) LU2MI_CHECK(hipMemcpyAsync(coef, c, sizeof(double) * m, hipMemcpyHostToDevice, s()));
+        hipLaunchKernelGGL(combine_kernel, dim3((n + 255) / 256), dim3(256), 0, s(), y, alpha,
+                           base ? base : (const double *)vec, (long)n, coef, m, n);
+        LU2MI_CHECK(hipGetLastError());
+        return LU2MI_OK;
+    }
+
+    // GMRES(m) on (LU)^-1 A d = (LU)^-1 r, left preconditioned; d returned in dout
+    int gmres(const double *r, double *dout, int m, double rtol, int *iters)
+    {
+        // V_0 .. V_m live in vec[0..m], w in vec[m+1]
+        double *w = V(m + 1);
+        LU2MI_CHECK(hipMemcpyAsync(V(0), r, sizeof(double) * n, hipMemcpyDeviceToDevice, s()));
+        LU2MI_TRY(lu_solve(V(0)));
+        double a[2];
+        LU2MI_TRY(norms(V(0), a));
+        const double beta = a[1];
+        if (beta == 0.0) {
+            LU2MI_CHECK(hipMemsetAsync(dout, 0, sizeof(double) * n, s()));
+            *iters = 0;
+            return LU2MI_OK;
+        }
+        double inv = 1.0 / beta;
+        LU2MI_TRY(combine(V(0), inv, nullptr, 0));
+        std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), h(m + 1);
+        g[0] = beta;
+        int k = 0;
+        for (; k < m; k++) {
+            hipLaunchKernelGGL(gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s(), A, lda, n, V(k), w, 0,
+                               (double *)nullptr);
+            LU2MI_CHECK(hipGetLastError());
+            LU2MI_TRY(lu_solve(w));
+            // classical Gram-Schmidt twice (batched dots)
+            for (int q = 0; q <= k; q++) h[q] = 0.0;
+            for (int pass = 0; pass < 2; pass++) {
+                std::vector<double> d(k + 1), c(k + 1);
+                LU2MI_TRY(dots(w, k + 1, d.data()));
+                for (int q = 0; q <= k; q++) {
+                    h[q] += d[q];
+                    c[q] = -d[q];
+                }
+                LU2MI_TRY(combine(w, 1.0, c.data(), k + 1));
+            }
+            LU2MI_TRY(norms(w, a));
+            h[k + 1] = a[1];
+            // V_{k+1} = w / h[k+1]
+            LU2MI_CHECK(hipMemcpyAsync(V(k + 1), w, sizeof(double) * n, hipMemcpyDeviceToDevice, s()));
+            if (h[k + 1] != 0.0) LU2MI_TRY(combine(V(k + 1), 1.0 / h[k + 1], nullptr, 0));
+            // apply the previous rotations, then a new one
+            for (int q = 0; q < k; q++) {
+                const double t0 = cs[q] * h[q] + sn[q] * h[q + 1];
+                h[q + 1] = -sn[q] * h[q] + cs[q] * h[q + 1];
+                h[q] = t0;
+            }
+            const double rr = hypot(h[k], h[k + 1]);
+            cs[k] = rr == 0.0 ? 1.0 : h[k] / rr;
+            sn[k] = rr == 0.0 ? 0.0 : h[k + 1] / rr;
+            h[k] = rr;
+            h[k + 1] = 0.0;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = cs[k] * g[k];
+            for (int q = 0; q <= k; q++) H[(size_t)q * m + k] = h[q];
+            if (fabs(g[k + 1]) <= rtol * beta) {
+                k++;
+                break;
+            }
+        }
+        // back-substitute the k x k triangle, d = V y
+        std::vector<double> y(k);
+        for (int q = k - 1; q >= 0; q--) {
+            double acc = g[q];
+            for (int p = q + 1; p < k; p++) acc -= H[(size_t)q * m + p] * y[p];
+            y[q] = acc / H[(size_t)q * m + q];
+        }
+        LU2MI_CHECK(hipMemsetAsync(dout, 0, sizeof(double) * n, s()));
+        LU2MI_TRY(combine(dout, 0.0, y.data(), k));
+        *iters = k;
+        return LU2MI_OK;
+    }
+};
+}  // namespace
+
+extern "C" int lu2mi_solve_mixed(lu2mi_ctx *ctx, int n, const double *dA, long lda, int nb, int prec, double *dx,
+                                 int maxiter, double target, lu2mi_ir_info *info)
+{
+    if (!ctx || !dA || !dx || n <= 0 || lda < n + 1) {
+        lu2mi_set_error_msg("solve_mixed: bad arguments");
+        return LU2MI_ERR_ARG;
+    }
+    if (prec != LU2MI_PREC_FP32) {
+        lu2mi_set_error_msg("solve_mixed: precision %d not provided", prec);
+        return LU2MI_ERR_UNSUPPORTED;
+    }
+    LU2MI_CHECK(hipSetDevice(ctx->device));
+    lu2mi_ir_info inf;
+    memset(&inf, 0, sizeof inf);
+    const int gm = 30;                                 // GMRES restart length
+    const long ldf = (long)n + 16;
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->m_af, &ctx->m_af_bytes, sizeof(float) * (size_t)n * ldf));
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->m_linv, &ctx->m_linv_bytes, sizeof(float) * (size_t)n * nb));
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->m_pairs, &ctx->m_pairs_bytes, sizeof(int) * 4 * (size_t)n));
+    // Krylov vectors V_0..V_gm, w, r, d, spare, |row| sums; then the reduction
+    // partials (NPART per dot, up to gm+1 dots) and the combine coefficients
+    const size_t npart = (size_t)NPART * (gm + 2);
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->m_vec, &ctx->m_vec_bytes,
+                            sizeof(double) * ((size_t)(gm + 6) * n + npart + (gm + 2))));
+    Mixed M;
+    M.ctx = ctx; M.n = n; M.nb = nb; M.A = dA; M.lda = lda; M.F = (float *)ctx->m_af; M.ldf = ldf;
+    M.linv_all = (float *)ctx->m_linv; M.pairs_all = (int *)ctx->m_pairs;
+    M.vec = (double *)ctx->m_vec;
+    M.part = M.vec + (size_t)(gm + 6) * n;
+    M.coef = M.part + npart;
+    double *r = M.V(gm + 2), *d = M.V(gm + 3), *xf64 = M.V(gm + 4);
+    (void)xf64;
+
+    auto t0 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(cvt_to_f32_kernel, dim3(n), dim3(256), 0, ctx->stream, dA, lda, n, M.F, ldf);
+    LU2MI_CHECK(hipGetLastError());
+    float *xf = (float *)d;   // the factorization's own FP32 solution
+    LU2MI_TRY(lu2mi_factor_f32(ctx, n, M.F, ldf, nb, ctx->pivmode, nullptr, xf, M.linv_all, M.pairs_all));
+    auto t1 = std::chrono::steady_clock::now();
+    inf.t_factor_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    // x0 = FP32 solution widened to FP64
+    {
+        std::vector<float> hx(n);
+        std::vector<double> hd(n);
+        LU2MI_CHECK(hipMemcpy(hx.data(), xf, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (int e = 0; e < n; e++) hd[e] = hx[e];
+        LU2MI_CHECK(hipMemcpy(dx, hd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    // ||A||_oo and ||b||_oo once
+    {
+        double *rowabs = M.V(gm + 5);
+        hipLaunchKernelGGL(gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, dA, lda, n, dx, r, 1, rowabs);
+        LU2MI_CHECK(hipGetLastError());
+        double a[2];
+        LU2MI_TRY(M.norms(rowabs, a));
+        M.anorm = a[0];
+        std::vector<double> hb(n);
+        LU2MI_CHECK(hipMemcpy2D(hb.data(), sizeof(double), dA + n, sizeof(double) * lda, sizeof(double), n,
+                                hipMemcpyDeviceToHost));
+        double bm = 0;
+        for (double v : hb) bm = fmax(bm, fabs(v));
+        M.bnorm = bm;
+    }
+    const double eps = 1.1102230246251565e-16;
+    auto scaled = [&](double rn, double xn) { return rn / (eps * (M.anorm * xn + M.bnorm) * n); };
+    double rn, xn;
+    LU2MI_TRY(M.residual(dx, r, &rn, &xn));
+    inf.resid_hpl2_initial = scaled(rn, xn);
+    double res = inf.resid_hpl2_initial, prev = INFINITY;
+    int it = 0, use_gmres = 0, gmres_inner = 0;
+    while (res > target && it < maxiter) {
+        if (!use_gmres && it >= 2 && res > 0.5 * prev) use_gmres = 1;   // classical refinement stalled
+        if (use_gmres) {
+            int k = 0;
+            LU2MI_TRY(M.gmres(r, d, gm, 1e-10, &k));
+            gmres_inner += k;
+        } else {
+            LU2MI_CHECK(hipMemcpyAsync(d, r, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+            LU2MI_TRY(M.lu_solve(d));
+        }
+        const double one = 1.0;
+        LU2MI_TRY(M.combine(dx, 1.0, &one, 1, d));   // x += d
+        it++;
+        prev = res;
+        LU2MI_TRY(M.residual(dx, r, &rn, &xn));
+        res = scaled(rn, xn);
+        if (!(res == res)) break;   // NaN: give up
+    }
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    auto t2 = std::chrono::steady_clock::now();
+    inf.t_refine_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    inf.iterations = it;
+    inf.gmres_used = use_gmres;
+    inf.gmres_inner = gmres_inner;
+    inf.resid_hpl2 = res;
+    if (info) *info = inf;
+    int herr = 0;
+    LU2MI_CHECK(hipMemcpy(&herr, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) {
+        lu2mi_set_error_msg("device wait timed out");
+        return LU2MI_ERR_DEVICE;
+    }
+    if (!(res <= target)) {
+        lu2mi_set_error_msg("refinement reached scaled residual %.3g after %d iterations", res, it);
+        return LU2MI_ERR_NOCONV;
+    }
+    return LU2MI_OK;
+}
+
+// FP32 GEMM on device pointers (test and tuning entry)
+extern "C" int lu2mi_sgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, float alpha, const float *dA, long lda,
+                               const float *dB, long ldb, float beta, float *dC, long ldc)
+{
+    if (!ctx) return LU2MI_ERR_ARG;
+    LU2MI_CHECK(hipSetDevice(ctx->device));
+    GemmDesc g;
+    g.M = m; g.N = n; g.K = k; g.alpha = alpha; g.beta = beta;
+    g.A = dA; g.lda = lda; g.B = dB; g.ldb = ldb; g.C = dC; g.ldc = ldc;
+    LU2MI_TRY(lu2mi_gemm_f32(g, ctx->stream));
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    return LU2MI_OK;
+}

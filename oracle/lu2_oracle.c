@@ -244,6 +244,16 @@ static void cm_recursive(int n, double *a, long lda, int m, int *pv, int i, int 
     for (int ii = i + h; ii < i + m; ii++) scale_row(a, lda, ii, 1.0 / A_(ii, ii), i, i + h);
 }
 
+/* lu2tlib.c:1130-1137 cm_column_decomposition_recursive: the panel of m
+ * columns starting at column i (rows [i, n)); restated on row-major storage
+ * (the column-major copy of lu2.c:1072-1085 is pure data movement). */
+int lu2o_cm_panel(int n, double *a, long lda, int m, int *pv, int i, int pivmode)
+{
+    if (m < 2 || (m & (m - 1)) || i < 0 || i + m > n || lda < i + m) return -1;
+    cm_recursive(n, a, lda, m, pv, i, pivmode);
+    return 0;
+}
+
 /* lu2.c:370-465 solve_triangle_for_unit_mat_recursive: b := L^-1 b for the
  * unit-lower part of the m x m block l (b = I on entry at the top level). */
 static void tri_inv_rec(int m, const double *l, long ldl, double *b, long ldb)

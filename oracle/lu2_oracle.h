@@ -48,6 +48,9 @@ int    lu2o_lumcolumn(int n, double *a, long lda, int nb, int pivmode,
  * with the leaf pivot rule of lu2tlib.c:1050 when pivmode is COMPAT). */
 int    lu2o_gepp_unblocked(int n, double *a, long lda, int nb, int pivmode,
                            int *piv, double *x);
+/* lu2tlib.c:1130-1137 cm_column_decomposition_recursive on the m-column panel
+ * at column i of a row-major matrix (rows [i, n)); pv[k] = global pivot row. */
+int    lu2o_cm_panel(int n, double *a, long lda, int m, int *pv, int i, int pivmode);
 /* lu2.c:110-116 backward_sub on the unit-upper factor; x <- column n. */
 void   lu2o_backward_sub(int n, const double *a, long lda, double *x);
 

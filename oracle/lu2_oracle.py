@@ -47,6 +47,8 @@ def lib() -> ctypes.CDLL:
         L.lu2o_emax.restype = c_double
         L.lu2o_residuals.argtypes = [c_int, _dp, c_long, _dp, _dp]
         L.lu2o_factor_solve.argtypes = [c_int, _dp, c_long, c_int, _ip, _dp]
+        L.lu2o_cm_panel.argtypes = [c_int, _dp, c_long, c_int, _ip, c_int, c_int]
+        L.lu2o_cm_panel.restype = c_int
         L.lu2o_kat_identity.argtypes = [c_int]
         L.lu2o_kat_identity.restype = ctypes.c_long
         _lib = L
@@ -125,6 +127,17 @@ def factor_solve(f: np.ndarray, nb: int, piv: np.ndarray, r: np.ndarray) -> np.n
     out = np.array(r, dtype=np.float64, copy=True)
     lib().lu2o_factor_solve(n, np.ascontiguousarray(f), lda, nb, np.ascontiguousarray(piv, np.int32), out)
     return out
+
+
+def cm_panel(a_rm: np.ndarray, m: int, i: int, pivmode: int):
+    """lu2tlib.c:1130 on columns [i, i+m) of a row-major n x lda matrix."""
+    f = np.array(a_rm, dtype=np.float64, copy=True, order="C")
+    n, lda = f.shape
+    pv = np.zeros(m, np.int32)
+    rc = lib().lu2o_cm_panel(n, f, lda, m, pv, i, pivmode)
+    if rc:
+        raise ValueError(f"lu2o_cm_panel rc={rc}")
+    return pv, f
 
 
 def kat_identity(nmat: int) -> int:

@@ -30,6 +30,7 @@ def lib():
     L.gdr_check_and_restart.argtypes = [dp, dp, dp]
     L.reset_gdr.argtypes = [c_int, dp, c_int, dp, c_int]
     L.lumcolumn.argtypes = [c_int, dp, dp, c_int, dp, dp, c_int]
+    L.cm_column_decomposition_recursive.argtypes = [c_int, dp, c_int, dp, c_int]
     return L
 
 
@@ -144,6 +145,25 @@ def test_lumcolumn_host_dropin(lib, oracle, n, nb, seed):
     assert np.array_equal(pd[n - nb:], pv[:nb])
     assert np.array_equal(b, xd)
     assert np.array_equal(a[:, : n + 1], fd[:, : n + 1])
+
+
+@pytest.mark.parametrize("n,m,i", [(300, 16, 0), (1000, 16, 64), (777, 32, 100)])
+def test_cm_column_decomposition_recursive(lib, oracle, n, m, i):
+    """lu2tlib.c:1130 on the column-major awork of lu2.c:1072-1085: same
+    pivots as the oracle's restatement, values to rounding (lu2's rule)."""
+    rng = np.random.default_rng(n + m + i)
+    rm = np.zeros((n, i + m + 3))
+    rm[:, i:i + m] = rng.uniform(-0.5, 0.5, (n, m))
+    cm = np.zeros((m, n))
+    cm[:, :] = rm[:, i:i + m].T
+    cm_in = cm.copy()
+    pv = np.zeros(m, np.int32)
+    lib.cm_column_decomposition_recursive(n, ptr(cm), m, ptr(pv), i)
+    po, fo = oracle.cm_panel(rm, m, i, oracle.PIV_COMPAT)
+    assert np.array_equal(pv, po)
+    want = fo[i:, i:i + m]
+    assert np.abs(cm[:, i:].T - want).max() <= 1e-12 * np.abs(want).max() * m
+    assert np.array_equal(cm[:, :i], cm_in[:, :i])     # rows above i untouched
 
 
 def test_driver_lu2mi(oracle):
