@@ -23,8 +23,7 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
 namespace {
-constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
-constexpr int ALD = BK + 1;
+constexpr int BM = 128, BN = 128, NT = 256;
 constexpr int GROUP_M = 8;
 constexpr int NXCD = 8;
 
@@ -53,44 +52,68 @@ __device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int
 //                 EPI_SET      C = acc            (alpha = 1, beta = 0: the TRSM products)
 enum { EPI_GENERAL = 0, EPI_SUB = 1, EPI_SET = 2 };
 
-template <bool EDGE, int EPI>
-__device__ __forceinline__ void gemm_f64_tile(const GemmDesc &g, const double *__restrict__ A,
-                                              const double *__restrict__ B, double *__restrict__ C,
-                                              int m0, int n0, int K, double (*As)[BM * ALD],
-                                              double (*Bs)[BK * BN])
+// Element-type traits.  BK is chosen so that one K stage is 16 bytes per
+// A row: the byte traffic, the LDS footprint and the 16-byte load mapping are
+// the same for FP64 (BK=16) and FP32 (BK=32).
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef d4_t acc_t;
+    typedef double2 vec_t;   // 16 bytes
+    static constexpr int BK = 16, EPV = 2;
+    static __device__ __forceinline__ acc_t step(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // lane l, register r of a 16x16 block holds row l/16 + 4r, column l%16
+    static __device__ __forceinline__ int out_row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+    typedef f4_t acc_t;
+    typedef float4 vec_t;
+    static constexpr int BK = 32, EPV = 4;
+    static __device__ __forceinline__ acc_t step(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // row 4*(l/16) + r, column l%16
+    static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T, bool EDGE, int EPI>
+__device__ __forceinline__ void gemm_tile(const GemmDesc &g, const T *__restrict__ A, const T *__restrict__ B,
+                                          T *__restrict__ C, int m0, int n0, int K,
+                                          T (*As)[BM * (Mfma<T>::BK + 1)], T (*Bs)[Mfma<T>::BK * BN])
 {
+    typedef Mfma<T> F;
+    typedef typename F::vec_t V;
+    constexpr int BKT = F::BK, EPV = F::EPV, AL = BKT + 1;
+    constexpr int BCH = BN / EPV;                 // 16-byte chunks per B row
     const int M = g.M, N = g.N;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w >> 1, wn = w & 1;
     const long lda = g.lda, ldb = g.ldb;
 
-    double2 ra[4], rb[4];
+    V ra[4], rb[4];
     auto load_tiles = [&](int k0) {
 #pragma unroll
         for (int it = 0; it < 4; it++) {
             const int idx = t + it * NT;
-            const int ar = idx >> 3, ac = (idx & 7) * 2;
-            const int br = idx >> 6, bc = (idx & 63) * 2;
+            const int ar = idx >> 3, ac = (idx & 7) * EPV;
+            const int br = idx / BCH, bc = (idx % BCH) * EPV;
             if (!EDGE) {
-                ra[it] = *(const double2 *)(A + (long)(m0 + ar) * lda + k0 + ac);
-                rb[it] = *(const double2 *)(B + (long)(k0 + br) * ldb + n0 + bc);
+                ra[it] = *(const V *)(A + (long)(m0 + ar) * lda + k0 + ac);
+                rb[it] = *(const V *)(B + (long)(k0 + br) * ldb + n0 + bc);
             } else {
-                const int gr = m0 + ar, gc = k0 + ac;
-                double2 v = {0.0, 0.0};
-                if (gr < M) {
-                    const double *p = A + (long)gr * lda + gc;
-                    if (gc < K) v.x = p[0];
-                    if (gc + 1 < K) v.y = p[1];
+                T va[EPV], vb[EPV];
+                const int gr = m0 + ar, gk = k0 + br;
+#pragma unroll
+                for (int e = 0; e < EPV; e++) {
+                    const int gc = k0 + ac + e, gn = n0 + bc + e;
+                    va[e] = (gr < M && gc < K) ? A[(long)gr * lda + gc] : T(0);
+                    vb[e] = (gk < K && gn < N) ? B[(long)gk * ldb + gn] : T(0);
                 }
-                ra[it] = v;
-                const int gk = k0 + br, gn = n0 + bc;
-                double2 u = {0.0, 0.0};
-                if (gk < K) {
-                    const double *q = B + (long)gk * ldb + gn;
-                    if (gn < N) u.x = q[0];
-                    if (gn + 1 < N) u.y = q[1];
-                }
-                rb[it] = u;
+                ra[it] = *(V *)va;
+                rb[it] = *(V *)vb;
             }
         }
     };
@@ -98,41 +121,41 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmDesc &g, const double *_
 #pragma unroll
         for (int it = 0; it < 4; it++) {
             const int idx = t + it * NT;
-            const int ar = idx >> 3, ac = (idx & 7) * 2;
-            As[buf][ar * ALD + ac] = ra[it].x;
-            As[buf][ar * ALD + ac + 1] = ra[it].y;
-            const int br = idx >> 6, bc = (idx & 63) * 2;
-            *(double2 *)&Bs[buf][br * BN + bc] = rb[it];
+            const int ar = idx >> 3, ac = (idx & 7) * EPV;
+            const T *pa = (const T *)&ra[it];
+#pragma unroll
+            for (int e = 0; e < EPV; e++) As[buf][ar * AL + ac + e] = pa[e];
+            const int br = idx / BCH, bc = (idx % BCH) * EPV;
+            *(V *)&Bs[buf][br * BN + bc] = rb[it];
         }
     };
 
-    d4_t acc[4][4];
+    typename F::acc_t acc[4][4];
 #pragma unroll
     for (int mi = 0; mi < 4; mi++)
 #pragma unroll
-        for (int nj = 0; nj < 4; nj++) acc[mi][nj] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        for (int nj = 0; nj < 4; nj++) acc[mi][nj] = (typename F::acc_t){T(0), T(0), T(0), T(0)};
 
     auto compute = [&](int cur) {
-        const double *as = As[cur];
-        const double *bs = Bs[cur];
+        const T *as = As[cur];
+        const T *bs = Bs[cur];
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            double a[4], b[4];
+        for (int kk = 0; kk < BKT; kk += 4) {
+            T a[4], b[4];
 #pragma unroll
             for (int mi = 0; mi < 4; mi++)
-                a[mi] = as[(wm * 64 + mi * 16 + (lane & 15)) * ALD + kk + (lane >> 4)];
+                a[mi] = as[(wm * 64 + mi * 16 + (lane & 15)) * AL + kk + (lane >> 4)];
 #pragma unroll
             for (int nj = 0; nj < 4; nj++)
                 b[nj] = bs[(kk + (lane >> 4)) * BN + wn * 64 + nj * 16 + (lane & 15)];
 #pragma unroll
             for (int mi = 0; mi < 4; mi++)
 #pragma unroll
-                for (int nj = 0; nj < 4; nj++)
-                    acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mi], b[nj], acc[mi][nj], 0, 0, 0);
+                for (int nj = 0; nj < 4; nj++) acc[mi][nj] = F::step(a[mi], b[nj], acc[mi][nj]);
         }
     };
 
-    const int nk = (K + BK - 1) / BK;
+    const int nk = (K + BKT - 1) / BKT;
     if (nk > 0) {
         load_tiles(0);
         store_tiles(0);
@@ -140,7 +163,7 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmDesc &g, const double *_
         // steady state: no conditionals around the loads
         for (int kt = 0; kt < nk - 1; kt++) {
             const int cur = kt & 1;
-            load_tiles((kt + 1) * BK);
+            load_tiles((kt + 1) * BKT);
             compute(cur);
             store_tiles(cur ^ 1);
             __syncthreads();
@@ -148,30 +171,31 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmDesc &g, const double *_
         compute((nk - 1) & 1);
     }
 
-    const double alpha = g.alpha, beta = g.beta;
+    const T alpha = (T)g.alpha, beta = (T)g.beta;
 #pragma unroll
     for (int mi = 0; mi < 4; mi++)
 #pragma unroll
         for (int nj = 0; nj < 4; nj++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm * 64 + mi * 16 + (lane >> 4) + 4 * r;
+                const int row = m0 + wm * 64 + mi * 16 + F::out_row(lane, r);
                 const int col = n0 + wn * 64 + nj * 16 + (lane & 15);
                 if (!EDGE || (row < M && col < N)) {
-                    double *cp = C + (long)row * g.ldc + col;
-                    const double v = acc[mi][nj][r];
+                    T *cp = C + (long)row * g.ldc + col;
+                    const T v = acc[mi][nj][r];
                     if (EPI == EPI_SUB) *cp = *cp - v;
                     else if (EPI == EPI_SET) *cp = v;
-                    else *cp = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * *cp);
+                    else *cp = (beta == T(0)) ? alpha * v : alpha * v + beta * *cp;
                 }
             }
 }
 
-template <bool VEC, int EPI, int TAG>
-__global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, int tiles_m, int tiles_n)
+template <typename T, bool VEC, int EPI, int TAG>
+__global__ __launch_bounds__(NT) void gemm_kernel(GemmDesc g, int tiles_m, int tiles_n)
 {
-    __shared__ double As[2][BM * ALD];
-    __shared__ double Bs[2][BK * BN];
+    constexpr int BKT = Mfma<T>::BK;
+    __shared__ T As[2][BM * (BKT + 1)];
+    __shared__ T Bs[2][BKT * BN];
 
     const int total = tiles_m * tiles_n;
     const int tile = xcd_remap(blockIdx.x, total);
@@ -179,142 +203,20 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmDesc g, int tiles_m, i
     tile_coords(tile, tiles_m, tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const long bz = blockIdx.y;
-    const double *A = (const double *)g.A + bz * g.sA;
-    const double *B = (const double *)g.B + bz * g.sB;
-    double *C = (double *)g.C + bz * g.sC;
+    const T *A = (const T *)g.A + bz * g.sA;
+    const T *B = (const T *)g.B + bz * g.sB;
+    T *C = (T *)g.C + bz * g.sC;
     const int K = g.a_lower ? min(g.K, m0 + BM) : g.K;
-    const bool interior = VEC && (m0 + BM <= g.M) && (n0 + BN <= g.N) && (K % BK == 0);
-    if (interior) gemm_f64_tile<false, EPI>(g, A, B, C, m0, n0, K, As, Bs);
-    else gemm_f64_tile<true, EPI>(g, A, B, C, m0, n0, K, As, Bs);
-}
-
-// ---------------------------------------------------------------------------
-// FP32 path: v_mfma_f32_16x16x4f32 with FP32 accumulation.  Operand layout as
-// the FP64 instruction; the result is D[4*(l/16) + r][l%16].
-template <bool VEC>
-__global__ __launch_bounds__(NT) void gemm_f32_kernel(GemmDesc g, int tiles_m, int tiles_n)
-{
-    __shared__ float As[2][BM * ALD];
-    __shared__ float Bs[2][BK * BN];
-
-    const int total = tiles_m * tiles_n;
-    const int tile = xcd_remap(blockIdx.x, total);
-    int tm, tn;
-    tile_coords(tile, tiles_m, tiles_n, tm, tn);
-    const int m0 = tm * BM, n0 = tn * BN;
-    const long bz = blockIdx.y;
-    const float *__restrict__ A = (const float *)g.A + bz * g.sA;
-    const float *__restrict__ B = (const float *)g.B + bz * g.sB;
-    float *__restrict__ C = (float *)g.C + bz * g.sC;
-    const int M = g.M, N = g.N;
-    const int K = g.a_lower ? min(g.K, m0 + BM) : g.K;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wm = w >> 1, wn = w & 1;
-
-    float2 ra[4], rb[4];
-    auto load_tiles = [&](int k0) {
-#pragma unroll
-        for (int it = 0; it < 4; it++) {
-            const int idx = t + it * NT;
-            const int ar = idx >> 3, ac = (idx & 7) * 2;
-            const int gr = m0 + ar, gc = k0 + ac;
-            float2 v = {0.f, 0.f};
-            if (gr < M) {
-                const float *p = A + (long)gr * g.lda + gc;
-                if (VEC && gc + 1 < K) {
-                    v = *(const float2 *)p;
-                } else {
-                    if (gc < K) v.x = p[0];
-                    if (gc + 1 < K) v.y = p[1];
-                }
-            }
-            ra[it] = v;
-            const int br = idx >> 6, bc = (idx & 63) * 2;
-            const int gk = k0 + br, gn = n0 + bc;
-            float2 u = {0.f, 0.f};
-            if (gk < K) {
-                const float *q = B + (long)gk * g.ldb + gn;
-                if (VEC && gn + 1 < N) {
-                    u = *(const float2 *)q;
-                } else {
-                    if (gn < N) u.x = q[0];
-                    if (gn + 1 < N) u.y = q[1];
-                }
-            }
-            rb[it] = u;
-        }
-    };
-    auto store_tiles = [&](int buf) {
-#pragma unroll
-        for (int it = 0; it < 4; it++) {
-            const int idx = t + it * NT;
-            const int ar = idx >> 3, ac = (idx & 7) * 2;
-            As[buf][ar * ALD + ac] = ra[it].x;
-            As[buf][ar * ALD + ac + 1] = ra[it].y;
-            const int br = idx >> 6, bc = (idx & 63) * 2;
-            *(float2 *)&Bs[buf][br * BN + bc] = rb[it];
-        }
-    };
-
-    f4_t acc[4][4];
-#pragma unroll
-    for (int mi = 0; mi < 4; mi++)
-#pragma unroll
-        for (int nj = 0; nj < 4; nj++) acc[mi][nj] = (f4_t){0.f, 0.f, 0.f, 0.f};
-
-    const int nk = (K + BK - 1) / BK;
-    if (nk > 0) {
-        load_tiles(0);
-        store_tiles(0);
-        __syncthreads();
-        for (int kt = 0; kt < nk; kt++) {
-            const int cur = kt & 1;
-            if (kt + 1 < nk) load_tiles((kt + 1) * BK);
-            const float *as = As[cur];
-            const float *bs = Bs[cur];
-#pragma unroll
-            for (int kk = 0; kk < BK; kk += 4) {
-                float a[4], b[4];
-#pragma unroll
-                for (int mi = 0; mi < 4; mi++)
-                    a[mi] = as[(wm * 64 + mi * 16 + (lane & 15)) * ALD + kk + (lane >> 4)];
-#pragma unroll
-                for (int nj = 0; nj < 4; nj++)
-                    b[nj] = bs[(kk + (lane >> 4)) * BN + wn * 64 + nj * 16 + (lane & 15)];
-#pragma unroll
-                for (int mi = 0; mi < 4; mi++)
-#pragma unroll
-                    for (int nj = 0; nj < 4; nj++)
-                        acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mi], b[nj],
-                                                                          acc[mi][nj], 0, 0, 0);
-            }
-            if (kt + 1 < nk) store_tiles(cur ^ 1);
-            __syncthreads();
-        }
-    }
-
-    const float alpha = (float)g.alpha, beta = (float)g.beta;
-#pragma unroll
-    for (int mi = 0; mi < 4; mi++)
-#pragma unroll
-        for (int nj = 0; nj < 4; nj++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = m0 + wm * 64 + mi * 16 + 4 * (lane >> 4) + r;
-                const int col = n0 + wn * 64 + nj * 16 + (lane & 15);
-                if (row < M && col < N) {
-                    float *cp = C + (long)row * g.ldc + col;
-                    const float v = acc[mi][nj][r];
-                    *cp = (beta == 0.f) ? alpha * v : fmaf(alpha, v, beta * *cp);
-                }
-            }
+    const bool interior = VEC && (m0 + BM <= g.M) && (n0 + BN <= g.N) && (K % BKT == 0);
+    if (interior) gemm_tile<T, false, EPI>(g, A, B, C, m0, n0, K, As, Bs);
+    else gemm_tile<T, true, EPI>(g, A, B, C, m0, n0, K, As, Bs);
 }
 
 // ---------------------------------------------------------------------------
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 template <typename T>
-static int launch_gemm(const GemmDesc &g, hipStream_t s, bool is_f64)
+static int launch_gemm(const GemmDesc &g, hipStream_t s)
 {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return LU2MI_OK;
     if (g.K < 0 || g.lda < (g.K > 0 ? g.K : 1) || g.ldb < g.N || g.ldc < g.N) {
@@ -328,33 +230,26 @@ static int launch_gemm(const GemmDesc &g, hipStream_t s, bool is_f64)
         lu2mi_set_error_msg("gemm: grid too large");
         return LU2MI_ERR_ARG;
     }
-    const size_t esz = sizeof(T);
-    (void)esz;
-    const bool vec = aligned16(g.A) && aligned16(g.B) && (g.lda % 2 == 0) && (g.ldb % 2 == 0) &&
-                     (g.sA % 2 == 0) && (g.sB % 2 == 0);
+    // 16-byte vector loads need 16-byte aligned rows of A and B
+    constexpr long EPV = Mfma<T>::EPV;
+    const bool vec = aligned16(g.A) && aligned16(g.B) && (g.lda % EPV == 0) && (g.ldb % EPV == 0) &&
+                     (g.sA % EPV == 0) && (g.sB % EPV == 0);
+    const int epi = (g.alpha == -1.0 && g.beta == 1.0) ? EPI_SUB : (g.alpha == 1.0 && g.beta == 0.0) ? EPI_SET
+                                                                                                   : EPI_GENERAL;
     dim3 grid((unsigned)total, (unsigned)g.batch), block(NT);
-    if (is_f64) {
-        const int epi = (g.alpha == -1.0 && g.beta == 1.0) ? EPI_SUB
-                        : (g.alpha == 1.0 && g.beta == 0.0) ? EPI_SET : EPI_GENERAL;
-        if (vec && epi == EPI_SUB && g.tag == 1)
-            hipLaunchKernelGGL((gemm_f64_kernel<true, EPI_SUB, 1>), grid, block, 0, s, g, tiles_m, tiles_n);
-        else if (vec && epi == EPI_SUB)
-            hipLaunchKernelGGL((gemm_f64_kernel<true, EPI_SUB, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
-        else if (vec && epi == EPI_SET)
-            hipLaunchKernelGGL((gemm_f64_kernel<true, EPI_SET, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
-        else if (vec)
-            hipLaunchKernelGGL((gemm_f64_kernel<true, EPI_GENERAL, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
-        else
-            hipLaunchKernelGGL((gemm_f64_kernel<false, EPI_GENERAL, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
-    } else {
-        const bool vecf = ((uintptr_t)g.A % 8 == 0) && ((uintptr_t)g.B % 8 == 0) &&
-                          (g.lda % 2 == 0) && (g.ldb % 2 == 0) && (g.sA % 2 == 0) && (g.sB % 2 == 0);
-        if (vecf) hipLaunchKernelGGL(gemm_f32_kernel<true>, grid, block, 0, s, g, tiles_m, tiles_n);
-        else hipLaunchKernelGGL(gemm_f32_kernel<false>, grid, block, 0, s, g, tiles_m, tiles_n);
-    }
+    if (vec && epi == EPI_SUB && g.tag == 1)
+        hipLaunchKernelGGL((gemm_kernel<T, true, EPI_SUB, 1>), grid, block, 0, s, g, tiles_m, tiles_n);
+    else if (vec && epi == EPI_SUB)
+        hipLaunchKernelGGL((gemm_kernel<T, true, EPI_SUB, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
+    else if (vec && epi == EPI_SET)
+        hipLaunchKernelGGL((gemm_kernel<T, true, EPI_SET, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
+    else if (vec)
+        hipLaunchKernelGGL((gemm_kernel<T, true, EPI_GENERAL, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
+    else
+        hipLaunchKernelGGL((gemm_kernel<T, false, EPI_GENERAL, 0>), grid, block, 0, s, g, tiles_m, tiles_n);
     LU2MI_CHECK(hipGetLastError());
     return LU2MI_OK;
 }
 
-int lu2mi_gemm_f64(const GemmDesc &g, hipStream_t s) { return launch_gemm<double>(g, s, true); }
-int lu2mi_gemm_f32(const GemmDesc &g, hipStream_t s) { return launch_gemm<float>(g, s, false); }
+int lu2mi_gemm_f64(const GemmDesc &g, hipStream_t s) { return launch_gemm<double>(g, s); }
+int lu2mi_gemm_f32(const GemmDesc &g, hipStream_t s) { return launch_gemm<float>(g, s); }

@@ -928,7 +928,7 @@ struct Factor {
             hipLaunchKernelGGL(block_pairs_kernel, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream,
                                ctx->d_piv, ctx->d_orig, i, nb, pairs, pairs + 2 * nb);
             LU2MI_CHECK(hipGetLastError());
-            const long ldw = (ncols + 1) & ~1L;
+            const long ldw = (ncols + 3) & ~3L;   // 16-byte rows for FP64 and FP32
             LU2MI_TRY(rowperm(pairs, pairs + 2 * nb, 2 * nb, i + nb, n + 1, 0, 0, n + 1, i, i + nb, wbuf, ldw, i,
                               i + nb));
             // U12 = inv(D^-1 L11) * staged rows   (lu2.c:511-512)
