@@ -118,6 +118,49 @@ int lu2mi_solve_mixed(lu2mi_ctx *ctx, int n, const double *dA, long lda, int nb,
 int lu2mi_sgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, float alpha, const float *dA, long lda,
                     const float *dB, long ldb, float beta, float *dC, long ldc);
 
+/* ---- batched uneven solve ----------------------------------------------
+ * No counterpart in the reference, which solves one system per run
+ * (lu2.c:1122-1144): many independent systems, each lu2's augmented
+ * row-major [A | B] (1 <= n <= 512, 1 <= nrhs <= 8, B in columns
+ * n..n+nrhs-1, leading dimension lda >= n+nrhs) at element offset a_off[s]
+ * of one FP64 device pool.  Each is factored with partial pivoting (LU of a
+ * working copy: dA is not modified) and solved; X[s] (n x nrhs, row-major)
+ * is written at offset x_off[s] of the solution pool.  One persistent
+ * launch serves all systems, longest first, from a queue; solves after the
+ * first replay a captured HIP graph.  prec FP64: plain FP64.  prec FP32:
+ * FP32 factors, FP64 classical refinement against dA until the HPL-2 scaled
+ * residual is at most target or maxiter steps are done. */
+#define LU2MI_BATCH_EAGER 1     /* flag: launch directly instead of the graph */
+
+typedef struct lu2mi_batch lu2mi_batch;
+typedef struct {
+    double max_resid_hpl2;      /* max over systems and right-hand sides (NaN if any is NaN) */
+    int    max_iterations;      /* refinement steps, 0 in the FP64 form */
+    int    n_failed;            /* systems above target (FP32) or above 16 (FP64) */
+    double t_ms;                /* device time of the launch (events around it) */
+    int    nt, wp_small, wp_large, split;  /* launch shape in use: threads per system,
+                                              panel width for n <= split and above */
+} lu2mi_batch_info;
+
+int lu2mi_batch_create(lu2mi_ctx *ctx, int nsys, const int *n, const int *nrhs, const long *a_off,
+                       const long *lda, const long *x_off, int prec, int maxiter, double target,
+                       lu2mi_batch **plan);
+int lu2mi_batch_destroy(lu2mi_batch *plan);
+/* per-system drand48 streams: system s is drawn row by row after
+ * srand48(base_seed + s), entries drand48()-0.5 (lu2.c:70-85's rule with
+ * nrhs B columns); padding columns are zeroed */
+int lu2mi_batch_generate(lu2mi_batch *plan, double *dA, long base_seed);
+/* autotuner: times every launch shape (threads per system 256/512, panel
+ * width 16/32 per size class) on this batch and keeps the fastest */
+int lu2mi_batch_tune(lu2mi_batch *plan, const double *dA, double *dX, int reps);
+int lu2mi_batch_set_config(lu2mi_batch *plan, int nt, int wp_small, int wp_large, int split);
+int lu2mi_batch_get_config(lu2mi_batch *plan, int *out4);
+int lu2mi_batch_solve(lu2mi_batch *plan, const double *dA, double *dX, int flags, lu2mi_batch_info *info);
+/* per-system HPL-2 residuals, refinement steps and pivot sequences (piv:
+ * the systems' sequences concatenated in system order, piv[j] = row
+ * exchanged with row j at step j); any pointer may be NULL */
+int lu2mi_batch_results(lu2mi_batch *plan, double *resid, int *iters, int *piv);
+
 /* Timing of the last lu2mi_lumcolumn_dev with LU2MI_OPT_TIMING on:
  * out[0]=total ms of trailing-update GEMMs (HIP events on the library
  * stream), out[1]=number of launches, out[2]=their algorithmic flops,

@@ -10,6 +10,8 @@ Mirrors the reference driver's operations (lu2.c):
   lumcolumn    (lu2.c:1122)    -> Context.lumcolumn_dev / solve()
   mydgemm      (lu2lib.c:2079) -> Context.dgemm_dev
   showresult   (lu2.c:45-60)   -> Context.residual_dev
+and the extensions: Context.solve_mixed (FP32 factors + FP64 refinement) and
+Batch (many independent systems of uneven size in one graph-captured launch).
 """
 from __future__ import annotations
 
@@ -28,6 +30,9 @@ PREC_FP32 = 1
 OPT_PIVMODE = 1
 OPT_LEAF_W = 2
 OPT_TIMING = 3
+BATCH_EAGER = 1
+BATCH_NMAX = 512
+BATCH_RHSMAX = 8
 
 _lib = None
 
@@ -41,6 +46,16 @@ class IRInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("gmres_used", ctypes.c_int), ("gmres_inner", ctypes.c_int),
                 ("resid_hpl2_initial", ctypes.c_double), ("resid_hpl2", ctypes.c_double),
                 ("t_factor_ms", ctypes.c_double), ("t_refine_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchInfo(ctypes.Structure):
+    """lu2mi_batch_info (include/lu2mi.h)."""
+    _fields_ = [("max_resid_hpl2", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("n_failed", ctypes.c_int), ("t_ms", ctypes.c_double), ("nt", ctypes.c_int),
+                ("wp_small", ctypes.c_int), ("wp_large", ctypes.c_int), ("split", ctypes.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -79,6 +94,15 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
                                       ctypes.POINTER(IRInfo)]),
         "lu2mi_sgemm_dev": (c_int, [vp, c_int, c_int, c_int, ctypes.c_float, vp, c_long, vp, c_long,
                                     ctypes.c_float, vp, c_long]),
+        "lu2mi_batch_create": (c_int, [vp, c_int, vp, vp, vp, vp, vp, c_int, c_int, c_double,
+                                       ctypes.POINTER(vp)]),
+        "lu2mi_batch_destroy": (c_int, [vp]),
+        "lu2mi_batch_generate": (c_int, [vp, vp, c_long]),
+        "lu2mi_batch_tune": (c_int, [vp, vp, vp, c_int]),
+        "lu2mi_batch_set_config": (c_int, [vp, c_int, c_int, c_int, c_int]),
+        "lu2mi_batch_get_config": (c_int, [vp, vp]),
+        "lu2mi_batch_solve": (c_int, [vp, vp, vp, c_int, ctypes.POINTER(BatchInfo)]),
+        "lu2mi_batch_results": (c_int, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -221,3 +245,76 @@ def solve(a_aug: np.ndarray, nb: int, pivmode: int = PIV_GEPP, ctx: Context | No
     finally:
         if own:
             ctx.close()
+
+
+class Batch:
+    """A plan for many independent systems of uneven size (lu2mi_batch_*).
+
+    System s is the augmented row-major [A | B] of n[s] rows and
+    n[s] + nrhs[s] columns at element offset a_off[s] of one FP64 pool, with
+    leading dimension lda[s]; its solution (n x nrhs, row-major) goes to
+    element offset x_off[s] of the solution pool.  By default the systems
+    are packed back to back (lda = n + nrhs)."""
+
+    def __init__(self, ctx: Context, n, nrhs, prec: int = PREC_FP64, maxiter: int = 10,
+                 target: float = 1.0, lda=None, a_off=None, x_off=None):
+        self.ctx = ctx
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
+        ns = len(self.n)
+        assert len(self.nrhs) == ns
+        self.lda = np.ascontiguousarray(self.n + self.nrhs if lda is None else lda, dtype=np.int64)
+        sizes = self.n.astype(np.int64) * self.lda
+        self.a_off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]) if a_off is None
+                                          else a_off, dtype=np.int64)
+        xs = self.n.astype(np.int64) * self.nrhs
+        self.x_off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(xs)[:-1]]) if x_off is None
+                                          else x_off, dtype=np.int64)
+        self.a_elems = int((self.a_off + sizes).max()) if ns else 0
+        self.x_elems = int((self.x_off + xs).max()) if ns else 0
+        self.prec = prec
+        h = ctypes.c_void_p()
+        _check(_lib.lu2mi_batch_create(ctx.handle, ns, self.n.ctypes.data, self.nrhs.ctypes.data,
+                                       self.a_off.ctypes.data, self.lda.ctypes.data, self.x_off.ctypes.data,
+                                       prec, maxiter, target, ctypes.byref(h)))
+        self.handle = h.value
+
+    def close(self) -> None:
+        if self.handle:
+            _check(_lib.lu2mi_batch_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def generate(self, dA: DeviceBuffer, base_seed: int) -> None:
+        _check(_lib.lu2mi_batch_generate(self.handle, dA.ptr, base_seed))
+
+    def tune(self, dA: DeviceBuffer, dX: DeviceBuffer, reps: int = 2) -> tuple:
+        _check(_lib.lu2mi_batch_tune(self.handle, dA.ptr, dX.ptr, reps))
+        return self.config()
+
+    def set_config(self, nt: int, wp_small: int, wp_large: int, split: int) -> None:
+        _check(_lib.lu2mi_batch_set_config(self.handle, nt, wp_small, wp_large, split))
+
+    def config(self) -> tuple:
+        out = np.zeros(4, dtype=np.int32)
+        _check(_lib.lu2mi_batch_get_config(self.handle, out.ctypes.data))
+        return tuple(int(v) for v in out)
+
+    def solve(self, dA: DeviceBuffer, dX: DeviceBuffer, eager: bool = False) -> dict:
+        info = BatchInfo()
+        _check(_lib.lu2mi_batch_solve(self.handle, dA.ptr if dA else None, dX.ptr if dX else None,
+                                      BATCH_EAGER if eager else 0, ctypes.byref(info)))
+        return info.as_dict()
+
+    def results(self):
+        ns = len(self.n)
+        resid = np.zeros(ns)
+        iters = np.zeros(ns, dtype=np.int32)
+        piv = np.zeros(max(1, int(self.n.sum())), dtype=np.int32)
+        _check(_lib.lu2mi_batch_results(self.handle, resid.ctypes.data, iters.ctypes.data, piv.ctypes.data))
+        return resid, iters, piv[:int(self.n.sum())]

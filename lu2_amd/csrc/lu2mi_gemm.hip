@@ -18,9 +18,7 @@
 // XCDs) owns a contiguous run of tiles in GROUP_M-row bands, which keeps the
 // shared A rows and B columns in that XCD's L2.
 #include "lu2mi_internal.h"
-
-typedef double d4_t __attribute__((ext_vector_type(4)));
-typedef float f4_t __attribute__((ext_vector_type(4)));
+#include "lu2mi_dev.h"
 
 namespace {
 constexpr int BM = 128, BN = 128, NT = 256;
@@ -51,33 +49,6 @@ __device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int
 //                 EPI_SUB      C = C - acc        (alpha = -1, beta = 1: the LU updates)
 //                 EPI_SET      C = acc            (alpha = 1, beta = 0: the TRSM products)
 enum { EPI_GENERAL = 0, EPI_SUB = 1, EPI_SET = 2 };
-
-// Element-type traits.  BK is chosen so that one K stage is 16 bytes per
-// A row: the byte traffic, the LDS footprint and the 16-byte load mapping are
-// the same for FP64 (BK=16) and FP32 (BK=32).
-template <typename T> struct Mfma;
-template <> struct Mfma<double> {
-    typedef d4_t acc_t;
-    typedef double2 vec_t;   // 16 bytes
-    static constexpr int BK = 16, EPV = 2;
-    static __device__ __forceinline__ acc_t step(double a, double b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    // lane l, register r of a 16x16 block holds row l/16 + 4r, column l%16
-    static __device__ __forceinline__ int out_row(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
-template <> struct Mfma<float> {
-    typedef f4_t acc_t;
-    typedef float4 vec_t;
-    static constexpr int BK = 32, EPV = 4;
-    static __device__ __forceinline__ acc_t step(float a, float b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    // row 4*(l/16) + r, column l%16
-    static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
 
 template <typename T, bool EDGE, int EPI>
 __device__ __forceinline__ void gemm_tile(const GemmDesc &g, const T *__restrict__ A, const T *__restrict__ B,

@@ -29,6 +29,7 @@
 // in lu2.c:28), b in column n.  Row-major rows are contiguous, so row swaps
 // and scalings are coalesced along rows.
 #include "lu2mi_internal.h"
+#include "lu2mi_dev.h"
 #include <limits.h>
 #include <stdarg.h>
 #include <stdlib.h>
@@ -36,30 +37,8 @@
 #include <chrono>
 #include <vector>
 
-// ---------------------------------------------------------------------------
-// drand48 on the device (lu2.c:70-85).  x_{k+1} = (a x_k + c) mod 2^48,
-// x_0 = (seed << 16) | 0x330E, value = x / 2^48.  Jump-ahead composes the
-// affine map; every element is bit-identical to glibc's sequential draws.
-namespace {
-constexpr unsigned long long M48 = (1ULL << 48) - 1;
-struct Lcg {
-    unsigned long long a, c;
-};
-__device__ __forceinline__ Lcg lcg_then(Lcg f, Lcg g)  // g after f
-{
-    return Lcg{(g.a * f.a) & M48, (g.a * f.c + g.c) & M48};
-}
-__device__ Lcg lcg_pow(unsigned long long steps)
-{
-    Lcg r{1, 0}, b{0x5DEECE66DULL, 0xBULL};
-    while (steps) {
-        if (steps & 1) r = lcg_then(r, b);
-        b = lcg_then(b, b);
-        steps >>= 1;
-    }
-    return r;
-}
-}  // namespace
+// drand48 on the device: jump-ahead generator in lu2mi_dev.h.
+using namespace lu2mi_rng;
 
 __global__ __launch_bounds__(256) void randomsetmat_kernel(double *A, long lda, int n,
                                                            unsigned long long x0)

@@ -496,3 +496,59 @@ long lu2o_kat_identity(int nmat)
     free(ia); free(b); free(c);
     return err;
 }
+
+/* ------------------------------------------------------------------ */
+/* Batched systems.  lu2 has no batched form; these restate, per system,
+ * the generator and the solve that the batched GPU form runs.           */
+
+/* rows x cols entries drawn row by row after srand48(seed), value
+ * drand48()-0.5: lu2.c:70-85's rule for an augmented matrix with
+ * cols - rows right-hand sides.  Columns [cols, lda) are set to 0. */
+void lu2o_random_aug(int rows, int cols, long seed, double *a, long lda)
+{
+    unsigned long long st;
+    lu2o_srand48(seed, &st);
+    for (long r = 0; r < rows; r++) {
+        for (long c = 0; c < cols; c++) A_(r, c) = lu2o_drand48(&st) - 0.5;
+        for (long c = cols; c < lda; c++) A_(r, c) = 0.0;
+    }
+}
+
+/* Textbook Gaussian elimination with partial pivoting on the augmented
+ * row-major [A | B] (n rows, n + nrhs columns): at step j the pivot is the
+ * first row of max |a[r][j]|, r >= j (a NaN never wins; a column with no
+ * comparable entry keeps row j), whole rows are swapped (LAPACK form), the
+ * multipliers a[r][j] * (1/a[j][j]) are stored below the diagonal and the
+ * rest of each row r > j, B included, is updated.  Then x = U^-1 y per
+ * right-hand side, x row-major n x nrhs.  piv[j] = row exchanged with row j. */
+int lu2o_gepp_solve(int n, int nrhs, double *a, long lda, int *piv, double *x)
+{
+    if (n <= 0 || nrhs <= 0 || lda < (long)n + nrhs) return -1;
+    const int nc = n + nrhs;
+    for (int j = 0; j < n; j++) {
+        int p = -1;
+        double best = -1.0;
+        for (int r = j; r < n; r++) {
+            double v = fabs(A_(r, j));
+            if (v > best) { best = v; p = r; }
+        }
+        if (p < 0) p = j;
+        piv[j] = p;
+        if (p != j)
+            for (int c = 0; c < nc; c++) { double t = A_(j, c); A_(j, c) = A_(p, c); A_(p, c) = t; }
+        double inv = 1.0 / A_(j, j);
+        for (int r = j + 1; r < n; r++) {
+            double l = A_(r, j) * inv;
+            A_(r, j) = l;
+            for (int c = j + 1; c < nc; c++) A_(r, c) -= l * A_(j, c);
+        }
+    }
+    for (int q = 0; q < nrhs; q++) {
+        for (int i = n - 1; i >= 0; i--) {
+            double s = A_(i, n + q);
+            for (int k = i + 1; k < n; k++) s -= A_(i, k) * x[(long)k * nrhs + q];
+            x[(long)i * nrhs + q] = s / A_(i, i);
+        }
+    }
+    return 0;
+}

@@ -76,6 +76,17 @@ void   lu2o_factor_solve(int n, const double *a, long lda, int nb,
  * B[i][j] = i*nmat+j, returns the number of entries with A*B != B. */
 long   lu2o_kat_identity(int nmat);
 
+/* Batched systems (lu2 has no batched form; the restated per-system rules).
+ * lu2o_random_aug: rows x cols entries drawn row by row after
+ *   srand48(seed), drand48()-0.5 (lu2.c:70-85's rule, cols-rows right-hand
+ *   sides); columns [cols, lda) set to 0.
+ * lu2o_gepp_solve: textbook GEPP on [A | B] (n x (n+nrhs)), whole-row swaps,
+ *   multipliers a*(1/d) below the diagonal, then x = U^-1 y (x: n x nrhs
+ *   row-major).  Pivot: first max |a|, NaN never wins, no comparable entry
+ *   keeps row j.  Returns 0, or -1 on bad arguments. */
+void   lu2o_random_aug(int rows, int cols, long seed, double *a, long lda);
+int    lu2o_gepp_solve(int n, int nrhs, double *a, long lda, int *piv, double *x);
+
 #ifdef __cplusplus
 }
 #endif

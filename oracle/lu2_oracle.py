@@ -49,6 +49,9 @@ def lib() -> ctypes.CDLL:
         L.lu2o_factor_solve.argtypes = [c_int, _dp, c_long, c_int, _ip, _dp]
         L.lu2o_cm_panel.argtypes = [c_int, _dp, c_long, c_int, _ip, c_int, c_int]
         L.lu2o_cm_panel.restype = c_int
+        L.lu2o_random_aug.argtypes = [c_int, c_int, c_long, _dp, c_long]
+        L.lu2o_gepp_solve.argtypes = [c_int, c_int, _dp, c_long, _ip, _dp]
+        L.lu2o_gepp_solve.restype = c_int
         L.lu2o_kat_identity.argtypes = [c_int]
         L.lu2o_kat_identity.restype = ctypes.c_long
         _lib = L
@@ -142,3 +145,39 @@ def cm_panel(a_rm: np.ndarray, m: int, i: int, pivmode: int):
 
 def kat_identity(nmat: int) -> int:
     return lib().lu2o_kat_identity(nmat)
+
+
+def random_aug(n: int, nrhs: int, seed: int, lda: int | None = None) -> np.ndarray:
+    """n x lda augmented [A | B] with nrhs right-hand sides, drawn row by row
+    after srand48(seed) (lu2.c:70-85's rule); padding columns 0."""
+    lda = n + nrhs if lda is None else lda
+    a = np.empty((n, lda))
+    lib().lu2o_random_aug(n, n + nrhs, seed, a, lda)
+    return a
+
+
+def gepp_solve(a_aug: np.ndarray, nrhs: int):
+    """Textbook GEPP + back substitution on [A | B]; returns (piv, x[n][nrhs],
+    factored copy)."""
+    f = np.ascontiguousarray(a_aug, dtype=np.float64).copy()
+    n = f.shape[0]
+    piv = np.zeros(n, dtype=np.int32)
+    x = np.zeros(n * nrhs)
+    rc = lib().lu2o_gepp_solve(n, nrhs, f, f.shape[1], piv, x)
+    if rc:
+        raise ValueError(f"lu2o_gepp_solve: {rc}")
+    return piv, x.reshape(n, nrhs), f
+
+
+def hpl2_multi(a_aug: np.ndarray, x: np.ndarray) -> float:
+    """Max over right-hand sides of the HPL-2 scaled residual
+    ||Ax-b||_oo / (eps (||A||_oo ||x||_oo + ||b||_oo) n), eps = 2^-53."""
+    n = a_aug.shape[0]
+    nrhs = x.shape[1]
+    A = a_aug[:, :n]
+    B = a_aug[:, n:n + nrhs]
+    r = B - A @ x
+    an = np.abs(A).sum(axis=1).max()
+    eps = 2.0 ** -53
+    res = np.abs(r).max(axis=0) / (eps * (an * np.abs(x).max(axis=0) + np.abs(B).max(axis=0)) * n)
+    return float(res.max())

@@ -158,3 +158,66 @@ def test_reference_limits_rejected(oracle):
         oracle.lumcolumn(a0, 48, 1)      # nb not a power of two
     with pytest.raises(ValueError):
         oracle.lumcolumn(a0, 64, 1)      # n % nb != 0
+
+
+# ---- batched systems: the per-system generator and GEPP solve -------------
+
+def test_random_aug_is_randomsetmat_rule(oracle):
+    """With one right-hand side the general generator is lu2's randomsetmat
+    (itself pinned against glibc's drand48 draws)."""
+    for n, seed in [(1, 5), (7, 1), (64, 12345)]:
+        a = oracle.random_aug(n, 1, seed, lda=n + 4)
+        assert np.array_equal(a[:, :n + 1], oracle.randomsetmat(n, seed)[:, :n + 1])
+        assert not a[:, n + 1:].any()
+    # more right-hand sides continue the same row-order stream
+    a3 = oracle.random_aug(4, 3, 9)           # 4 x 7 draws
+    flat = oracle.random_aug(1, 27, 9)[0]     # 28 draws in one row
+    assert np.array_equal(a3.ravel(), flat)
+
+
+@pytest.mark.parametrize("n,nrhs,seed", [(1, 1, 1), (2, 3, 2), (37, 4, 3), (200, 2, 4), (512, 8, 5)])
+def test_gepp_solve_matches_lapack(oracle, n, nrhs, seed):
+    import scipy.linalg as sl
+
+    a = oracle.random_aug(n, nrhs, seed)
+    piv, x, _ = oracle.gepp_solve(a, nrhs)
+    lu, ip = sl.lu_factor(a[:, :n])
+    assert np.array_equal(piv, ip)
+    xr = sl.lu_solve((lu, ip), a[:, n:])
+    assert np.abs(x - xr).max() / np.abs(xr).max() < 1e-11
+    assert oracle.hpl2_multi(a, x) < 1.0
+
+
+def test_gepp_solve_agrees_with_lu2_form(oracle):
+    """Standard GEPP pivots do not depend on lu2's stored form or blocking:
+    with one right-hand side the pivot sequence equals the lu2-form GEPP."""
+    n = 256
+    a = oracle.randomsetmat(n, 77)
+    piv, x, _ = oracle.gepp_solve(a[:, :n + 1], 1)
+    x2, p2, _ = oracle.gepp_unblocked(a, 64, oracle.PIV_GEPP)
+    assert np.array_equal(piv, p2)
+    assert np.abs(x[:, 0] - x2).max() / np.abs(x2).max() < 1e-12
+
+
+def test_gepp_solve_closed_forms(oracle):
+    # 2x2 with a pivot swap: [[1, 2 | 5], [3, 4 | 6]] -> x = (-4, 4.5)
+    a = np.array([[1.0, 2.0, 5.0], [3.0, 4.0, 6.0]])
+    piv, x, f = oracle.gepp_solve(a, 1)
+    assert list(piv) == [1, 1]
+    assert np.allclose(x[:, 0], [-4.0, 4.5], rtol=0, atol=1e-14)   # 1/3 rounds
+    assert f[1, 0] == 1.0 / 3.0   # multiplier a*(1/d)
+    # ties go to the first row; a NaN never wins
+    a = np.array([[1.0, 0.0, 1.0], [-1.0, 1.0, 0.0]])
+    piv, _, _ = oracle.gepp_solve(a, 1)
+    assert piv[0] == 0
+    a = np.hstack([np.array([[np.nan, 1.0, 0.0], [0.5, 0.0, 1.0], [0.25, 1.0, 1.0]]), np.ones((3, 1))])
+    piv, _, _ = oracle.gepp_solve(a, 1)
+    assert piv[0] == 1
+
+
+def test_hpl2_multi_detects_planted_error(oracle):
+    a = oracle.random_aug(100, 2, 8)
+    _, x, _ = oracle.gepp_solve(a, 2)
+    assert oracle.hpl2_multi(a, x) < 1.0
+    x[3, 1] += 1e-9
+    assert oracle.hpl2_multi(a, x) > 16
