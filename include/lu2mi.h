@@ -66,6 +66,12 @@ int lu2mi_memcpy_d2d(lu2mi_ctx *ctx, void *ddst, const void *dsrc, size_t bytes)
  * after srand48(seed).  Columns n+1..lda-1 are set to 0. */
 int lu2mi_randomsetmat(lu2mi_ctx *ctx, int n, int seed, double *dA, long lda);
 
+/* a[i][i] += sum_{j<n} |a[i][j]| for every row: lu2's random matrix made
+ * diagonally dominant, the test matrix of the 16-bit solves (no counterpart
+ * in the reference).  Each row sum is one device reduction; the result is
+ * within the rounding of that sum of the sequential one. */
+int lu2mi_make_diag_dominant(lu2mi_ctx *ctx, int n, double *dA, long lda);
+
 /* lu2.c:1122-1144 lumcolumn(n, a, b, nb, awork, pv, 1), GPU-resident:
  * factors dA in place (lu2's stored form, pivot rule from
  * LU2MI_OPT_PIVMODE) and writes x to dx (may be NULL).  dpiv (may be NULL)
@@ -92,6 +98,10 @@ int lu2mi_residual_dev(lu2mi_ctx *ctx, int n, const double *dA0, long lda,
 /* ---- mixed precision ---------------------------------------------------- */
 #define LU2MI_PREC_FP64 0
 #define LU2MI_PREC_FP32 1
+#define LU2MI_PREC_FP16 2   /* FP32 factors, trailing updates on FP16 operands */
+#define LU2MI_PREC_BF16 3   /* FP32 factors, trailing updates on BF16 operands */
+#define LU2MI_PREC_FP16X2 4 /* FP32 factors, trailing updates on FP16 operands split
+                               hi + lo (hi*hi + hi*lo + lo*hi, ~22 bits) */
 
 typedef struct {
     int    iterations;          /* refinement steps taken */
@@ -104,8 +114,11 @@ typedef struct {
 } lu2mi_ir_info;
 
 /* Solve lu2's augmented system (dA: FP64, b in column n, not modified) by an
- * LU of a copy in precision prec (FP32: FP32 MFMA trailing updates with FP32
- * accumulation; same pivot rule and factor form as lu2mi_lumcolumn_dev) and
+ * LU of an FP32 copy (same pivot rule and factor form as
+ * lu2mi_lumcolumn_dev) whose trailing updates run on the matrix cores with
+ * FP32 accumulation on FP32 operands (prec FP32) or on operands rounded to
+ * FP16 / BF16 under a power-of-two scale per 128-row block of L21 and per
+ * 128-column block of U12 (prec FP16 / BF16; nb a multiple of 64), and
  * FP64 iterative refinement -- classical, switching to GMRES-IR preconditioned
  * by the same factors when it stalls -- until the HPL-2 scaled residual is at
  * most target or maxiter steps are done.  x is written to dx (FP64).  Returns
@@ -113,6 +126,13 @@ typedef struct {
  * iterate). */
 int lu2mi_solve_mixed(lu2mi_ctx *ctx, int n, const double *dA, long lda, int nb, int prec,
                       double *dx, int maxiter, double target, lu2mi_ir_info *info);
+
+/* The 16-bit trailing update on its own: C -= L U, all FP32 row-major on the
+ * device (L: m x k, U: k x n), with L and U rounded to FP16 (fmt 1) or BF16
+ * (fmt 2), or split into FP16 hi + lo (fmt 3), under per-128-block
+ * power-of-two scales, FP32 accumulation.  k a multiple of 64. */
+int lu2mi_gemm16_dev(lu2mi_ctx *ctx, int fmt, int m, int n, int k, const float *dL, long ldl,
+                     const float *dU, long ldu, float *dC, long ldc);
 
 /* FP32 row-major GEMM on device pointers, C = alpha*A*B + beta*C (FP32 MFMA). */
 int lu2mi_sgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, float alpha, const float *dA, long lda,

@@ -27,6 +27,12 @@ PIV_COMPAT = 0
 PIV_GEPP = 1
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_FP16 = 2
+PREC_BF16 = 3
+PREC_FP16X2 = 4
+LOWP_FP16 = 1
+LOWP_BF16 = 2
+LOWP_FP16X2 = 3
 OPT_PIVMODE = 1
 OPT_LEAF_W = 2
 OPT_TIMING = 3
@@ -94,6 +100,8 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
                                       ctypes.POINTER(IRInfo)]),
         "lu2mi_sgemm_dev": (c_int, [vp, c_int, c_int, c_int, ctypes.c_float, vp, c_long, vp, c_long,
                                     ctypes.c_float, vp, c_long]),
+        "lu2mi_make_diag_dominant": (c_int, [vp, c_int, vp, c_long]),
+        "lu2mi_gemm16_dev": (c_int, [vp, c_int, c_int, c_int, c_int, vp, c_long, vp, c_long, vp, c_long]),
         "lu2mi_batch_create": (c_int, [vp, c_int, vp, vp, vp, vp, vp, c_int, c_int, c_double,
                                        ctypes.POINTER(vp)]),
         "lu2mi_batch_destroy": (c_int, [vp]),
@@ -192,6 +200,14 @@ class Context:
     def dgemm_dev(self, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, offs=(0, 0, 0)) -> None:
         _check(_lib.lu2mi_dgemm_dev(self.handle, m, n, k, alpha, dA.ptr + offs[0], lda,
                                     dB.ptr + offs[1], ldb, beta, dC.ptr + offs[2], ldc))
+
+    def make_diag_dominant(self, n: int, dA: DeviceBuffer, lda: int) -> None:
+        _check(_lib.lu2mi_make_diag_dominant(self.handle, n, dA.ptr, lda))
+
+    def gemm16_dev(self, fmt, m, n, k, dL, ldl, dU, ldu, dC, ldc) -> None:
+        """C -= L U with L, U rounded to FP16 (fmt 1) or BF16 (fmt 2) under
+        per-128-block power-of-two scales, FP32 accumulation."""
+        _check(_lib.lu2mi_gemm16_dev(self.handle, fmt, m, n, k, dL.ptr, ldl, dU.ptr, ldu, dC.ptr, ldc))
 
     def sgemm_dev(self, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc) -> None:
         _check(_lib.lu2mi_sgemm_dev(self.handle, m, n, k, alpha, dA.ptr, lda, dB.ptr, ldb, beta, dC.ptr, ldc))

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["lu2mi_gemm.hip", "lu2mi_lu.hip", "lu2mi_mixed.hip", "lu2mi_batched.hip", "lu2mi_api.hip"]
+SOURCES = ["lu2mi_gemm.hip", "lu2mi_lu.hip", "lu2mi_mixed.hip", "lu2mi_batched.hip", "lu2mi_lowp.hip", "lu2mi_api.hip"]
 LIB = os.path.join(HERE, "liblu2mi.so")
 DRIVER = os.path.join(HERE, "lu2mi")
 

@@ -21,6 +21,7 @@
 #include <vector>
 
 int lu2mi_randomsetmat_impl(lu2mi_ctx *ctx, int n, int seed, double *dA, long lda);
+int lu2mi_diag_dominant_impl(lu2mi_ctx *ctx, int n, double *dA, long lda);
 int lu2mi_residual_impl(lu2mi_ctx *ctx, int n, const double *dA0, long lda, const double *dx, double *out);
 
 #define CTX_OK(ctx)                                                   \
@@ -79,7 +80,7 @@ extern "C" int lu2mi_finalize(lu2mi_ctx *ctx)
     void *bufs[] = {ctx->d_piv, ctx->d_orig, ctx->d_invd, ctx->d_invd_f, ctx->d_pairs, ctx->d_linv,
                     ctx->d_tmp, ctx->d_w, ctx->d_slots, ctx->d_bar, ctx->d_err, ctx->d_y,
                     ctx->h_dev_a, ctx->h_dev_b, ctx->h_dev_c, ctx->m_af, ctx->m_linv, ctx->m_pairs,
-                    ctx->m_vec};
+                    ctx->m_vec, ctx->m_a16, ctx->m_b16, ctx->m_s16};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     free_timer(ctx->tgemm);
@@ -186,6 +187,12 @@ extern "C" int lu2mi_randomsetmat(lu2mi_ctx *ctx, int n, int seed, double *dA, l
 {
     CTX_OK(ctx);
     return lu2mi_randomsetmat_impl(ctx, n, seed, dA, lda);
+}
+
+extern "C" int lu2mi_make_diag_dominant(lu2mi_ctx *ctx, int n, double *dA, long lda)
+{
+    CTX_OK(ctx);
+    return lu2mi_diag_dominant_impl(ctx, n, dA, lda);
 }
 
 extern "C" int lu2mi_lumcolumn_dev(lu2mi_ctx *ctx, int n, double *dA, long lda, int nb, int *dpiv, double *dx)

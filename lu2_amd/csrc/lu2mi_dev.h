@@ -67,3 +67,30 @@ template <> struct Mfma<float> {
     }
     static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
+
+// ---------------------------------------------------------------------------
+// Tile order of the GEMM grids.  Workgroups go round-robin over the 8 XCDs;
+// xcd_remap gives each XCD a contiguous run of tile indices, and tile_coords
+// walks GROUP_M-row bands so that the A rows and B columns a run shares stay
+// in that XCD's L2.
+namespace lu2mi_tiles {
+constexpr int GROUP_M = 8;
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_remap(int bid, int total)
+{
+    const int q = total / NXCD, r = total % NXCD;
+    const int xcd = bid % NXCD, local = bid / NXCD;
+    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+__device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int &tm, int &tn)
+{
+    const int width = GROUP_M * tiles_n;
+    const int group = t / width;
+    const int first_m = group * GROUP_M;
+    const int gm = min(tiles_m - first_m, GROUP_M);
+    const int in = t - group * width;
+    tm = first_m + in % gm;
+    tn = in / gm;
+}
+}  // namespace lu2mi_tiles

@@ -22,27 +22,8 @@
 
 namespace {
 constexpr int BM = 128, BN = 128, NT = 256;
-constexpr int GROUP_M = 8;
-constexpr int NXCD = 8;
-
-__device__ __forceinline__ int xcd_remap(int bid, int total)
-{
-    const int q = total / NXCD, r = total % NXCD;
-    const int xcd = bid % NXCD, local = bid / NXCD;
-    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
-}
-
-__device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int &tm, int &tn)
-{
-    const int width = GROUP_M * tiles_n;
-    const int group = t / width;
-    const int first_m = group * GROUP_M;
-    const int gm = min(tiles_m - first_m, GROUP_M);
-    const int in = t - group * width;
-    tm = first_m + in % gm;
-    tn = in / gm;
-}
 }  // namespace
+using namespace lu2mi_tiles;
 
 // ---------------------------------------------------------------------------
 // Epilogue modes: EPI_GENERAL  C = alpha*acc + beta*C

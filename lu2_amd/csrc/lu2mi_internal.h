@@ -81,6 +81,12 @@ struct lu2mi_ctx {
     void *m_linv; size_t m_linv_bytes;
     void *m_pairs; size_t m_pairs_bytes;
     void *m_vec; size_t m_vec_bytes;
+    // 16-bit trailing updates: 0 off, LU2MI_LOWP_FP16 or LU2MI_LOWP_BF16; the
+    // scaled 16-bit copies of L21 and U12 and their block exponents
+    int lowp;
+    void *m_a16; size_t m_a16_bytes;
+    void *m_b16; size_t m_b16_bytes;
+    void *m_s16; size_t m_s16_bytes;
     // host staging buffers for the host-memory seams (mygdrdgemm, lumcolumn)
     void *h_dev_a; size_t h_dev_a_bytes;
     void *h_dev_b; size_t h_dev_b_bytes;
@@ -88,6 +94,14 @@ struct lu2mi_ctx {
 };
 
 int lu2mi_ctx_reserve(lu2mi_ctx *ctx, int n, int nb, int elem_bytes);
+
+#define LU2MI_LOWP_FP16 1
+#define LU2MI_LOWP_BF16 2
+#define LU2MI_LOWP_FP16X2 3   // FP16 hi + lo operand split, three products
+// C -= L U with the operands rounded to 16 bits under per-128-block
+// power-of-two scales, FP32 accumulation (stream-ordered, no sync)
+int lu2mi_update16(lu2mi_ctx *ctx, int fmt, int M, int N, int K, const float *L, long ldl, const float *U,
+                   long ldu, float *C, long ldc);
 int lu2mi_factor_f64(lu2mi_ctx *ctx, int n, double *dA, long lda, int nb, int pivmode,
                      int *dpiv_out, double *dx);
 int lu2mi_factor_f32(lu2mi_ctx *ctx, int n, float *dA, long lda, int nb, int pivmode,

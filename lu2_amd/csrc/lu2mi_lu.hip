@@ -59,6 +59,22 @@ __global__ __launch_bounds__(256) void randomsetmat_kernel(double *A, long lda, 
     for (long col = n + 1 + t; col < lda; col += 256) row[col] = 0.0;
 }
 
+// a[i][i] += sum_j |a[i][j]| over the n columns of A (diagonal dominance for
+// the 16-bit solves); one workgroup per row, fixed reduction order
+__global__ __launch_bounds__(256) void diag_dominant_kernel(double *A, long lda, int n)
+{
+    __shared__ double red[4];
+    const long r = blockIdx.x;
+    const int t = threadIdx.x;
+    const double *row = A + r * lda;
+    double s = 0.0;
+    for (int c = t; c < n; c += 256) s += fabs(row[c]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((t & 63) == 0) red[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) A[r * lda + r] += (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // ---------------------------------------------------------------------------
 // Leaf factorization: persistent kernel over rows [c, n), W columns; all
 // workgroups are co-resident and meet at a grid barrier once per column.
@@ -919,9 +935,14 @@ struct Factor {
                 KernelTimer &kt = ctx->tgemm;
                 const bool tm = ctx->timing_on && kt.used < kt.cap;
                 if (tm) LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used], ctx->stream));
-                LU2MI_TRY(gemm_call<T>(ctx, M, ncols, nb, -1.0, A + (long)(i + nb) * lda + i, lda,
-                                       A + (long)i * lda + i + nb, lda, 1.0, A + (long)(i + nb) * lda + i + nb,
-                                       lda, 0, 1));
+                if (sizeof(T) == 4 && ctx->lowp)   // operands rounded to FP16/BF16, FP32 accumulation
+                    LU2MI_TRY(lu2mi_update16(ctx, ctx->lowp, M, ncols, nb, (const float *)(A + (long)(i + nb) * lda + i),
+                                             lda, (const float *)(A + (long)i * lda + i + nb), lda,
+                                             (float *)(A + (long)(i + nb) * lda + i + nb), lda));
+                else
+                    LU2MI_TRY(gemm_call<T>(ctx, M, ncols, nb, -1.0, A + (long)(i + nb) * lda + i, lda,
+                                           A + (long)i * lda + i + nb, lda, 1.0, A + (long)(i + nb) * lda + i + nb,
+                                           lda, 0, 1));
                 if (tm) {
                     LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used + 1], ctx->stream));
                     kt.flops[kt.used] = 2.0 * M * (double)ncols * nb;
@@ -1063,6 +1084,18 @@ int lu2mi_randomsetmat_impl(lu2mi_ctx *ctx, int n, int seed, double *dA, long ld
     }
     const unsigned long long x0 = (((unsigned long long)(unsigned)seed) << 16) | 0x330EULL;
     hipLaunchKernelGGL(randomsetmat_kernel, dim3(n), dim3(256), 0, ctx->stream, dA, lda, n, x0);
+    LU2MI_CHECK(hipGetLastError());
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    return LU2MI_OK;
+}
+
+int lu2mi_diag_dominant_impl(lu2mi_ctx *ctx, int n, double *dA, long lda)
+{
+    if (!dA || n <= 0 || lda < n + 1) {
+        lu2mi_set_error_msg("make_diag_dominant: bad arguments");
+        return LU2MI_ERR_ARG;
+    }
+    hipLaunchKernelGGL(diag_dominant_kernel, dim3(n), dim3(256), 0, ctx->stream, dA, lda, n);
     LU2MI_CHECK(hipGetLastError());
     LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
     return LU2MI_OK;

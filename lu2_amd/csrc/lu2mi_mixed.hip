@@ -33,6 +33,17 @@ __global__ __launch_bounds__(256) void cvt_to_f32_kernel(const double *A, long l
     for (long c = threadIdx.x; c < ldf; c += 256) f[c] = (c <= n) ? (float)a[c] : 0.f;
 }
 
+// widen the factorization's FP32 solution, and gather column n (b) of A
+__global__ __launch_bounds__(256) void x0_b_kernel(const float *xf, const double *A, long lda, int n, double *x,
+                                                   double *b)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        x[i] = (double)xf[i];
+        b[i] = A[i * lda + n];
+    }
+}
+
 // One block of the forward solve: r[dst] = r[src] over the block's row
 // pairs (all reads first), rows of the block scaled by 1/d, then
 // r_block = Linv * r_block.  One workgroup.
@@ -339,9 +350,13 @@ extern "C" int lu2mi_solve_mixed(lu2mi_ctx *ctx, int n, const double *dA, long l
         lu2mi_set_error_msg("solve_mixed: bad arguments");
         return LU2MI_ERR_ARG;
     }
-    if (prec != LU2MI_PREC_FP32) {
+    if (prec != LU2MI_PREC_FP32 && prec != LU2MI_PREC_FP16 && prec != LU2MI_PREC_BF16 && prec != LU2MI_PREC_FP16X2) {
         lu2mi_set_error_msg("solve_mixed: precision %d not provided", prec);
         return LU2MI_ERR_UNSUPPORTED;
+    }
+    if (prec != LU2MI_PREC_FP32 && nb % 64) {
+        lu2mi_set_error_msg("solve_mixed: 16-bit updates need nb a multiple of 64 (nb=%d)", nb);
+        return LU2MI_ERR_ARG;
     }
     LU2MI_CHECK(hipSetDevice(ctx->device));
     lu2mi_ir_info inf;
@@ -362,38 +377,32 @@ extern "C" int lu2mi_solve_mixed(lu2mi_ctx *ctx, int n, const double *dA, long l
     M.vec = (double *)ctx->m_vec;
     M.part = M.vec + (size_t)(gm + 6) * n;
     M.coef = M.part + npart;
-    double *r = M.V(gm + 2), *d = M.V(gm + 3), *xf64 = M.V(gm + 4);
-    (void)xf64;
+    double *r = M.V(gm + 2), *d = M.V(gm + 3);
 
     auto t0 = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(cvt_to_f32_kernel, dim3(n), dim3(256), 0, ctx->stream, dA, lda, n, M.F, ldf);
     LU2MI_CHECK(hipGetLastError());
     float *xf = (float *)d;   // the factorization's own FP32 solution
-    LU2MI_TRY(lu2mi_factor_f32(ctx, n, M.F, ldf, nb, ctx->pivmode, nullptr, xf, M.linv_all, M.pairs_all));
+    ctx->lowp = (prec == LU2MI_PREC_FP16)   ? LU2MI_LOWP_FP16
+                : (prec == LU2MI_PREC_BF16) ? LU2MI_LOWP_BF16
+                : (prec == LU2MI_PREC_FP16X2) ? LU2MI_LOWP_FP16X2 : 0;
+    const int frc = lu2mi_factor_f32(ctx, n, M.F, ldf, nb, ctx->pivmode, nullptr, xf, M.linv_all, M.pairs_all);
+    ctx->lowp = 0;
+    LU2MI_TRY(frc);
     auto t1 = std::chrono::steady_clock::now();
     inf.t_factor_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    // x0 = FP32 solution widened to FP64
+    // x0 = FP32 solution widened to FP64; ||A||_oo and ||b||_oo once
     {
-        std::vector<float> hx(n);
-        std::vector<double> hd(n);
-        LU2MI_CHECK(hipMemcpy(hx.data(), xf, sizeof(float) * n, hipMemcpyDeviceToHost));
-        for (int e = 0; e < n; e++) hd[e] = hx[e];
-        LU2MI_CHECK(hipMemcpy(dx, hd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    }
-    // ||A||_oo and ||b||_oo once
-    {
-        double *rowabs = M.V(gm + 5);
+        double *rowabs = M.V(gm + 5), *bcol = M.V(gm + 4);
+        hipLaunchKernelGGL(x0_b_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, xf, dA, lda, n, dx, bcol);
+        LU2MI_CHECK(hipGetLastError());
         hipLaunchKernelGGL(gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, dA, lda, n, dx, r, 1, rowabs);
         LU2MI_CHECK(hipGetLastError());
         double a[2];
         LU2MI_TRY(M.norms(rowabs, a));
         M.anorm = a[0];
-        std::vector<double> hb(n);
-        LU2MI_CHECK(hipMemcpy2D(hb.data(), sizeof(double), dA + n, sizeof(double) * lda, sizeof(double), n,
-                                hipMemcpyDeviceToHost));
-        double bm = 0;
-        for (double v : hb) bm = fmax(bm, fabs(v));
-        M.bnorm = bm;
+        LU2MI_TRY(M.norms(bcol, a));
+        M.bnorm = a[0];
     }
     const double eps = 1.1102230246251565e-16;
     auto scaled = [&](double rn, double xn) { return rn / (eps * (M.anorm * xn + M.bnorm) * n); };

@@ -552,3 +552,14 @@ int lu2o_gepp_solve(int n, int nrhs, double *a, long lda, int *piv, double *x)
     }
     return 0;
 }
+
+/* a[i][i] += sum_{j<n} |a[i][j]| (sequential sum): lu2's random matrix made
+ * diagonally dominant for the 16-bit solves. */
+void lu2o_diag_dominant(int n, double *a, long lda)
+{
+    for (long r = 0; r < n; r++) {
+        double s = 0.0;
+        for (long c = 0; c < n; c++) s += fabs(A_(r, c));
+        A_(r, r) += s;
+    }
+}

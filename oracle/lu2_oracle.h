@@ -84,6 +84,8 @@ long   lu2o_kat_identity(int nmat);
  *   multipliers a*(1/d) below the diagonal, then x = U^-1 y (x: n x nrhs
  *   row-major).  Pivot: first max |a|, NaN never wins, no comparable entry
  *   keeps row j.  Returns 0, or -1 on bad arguments. */
+/* a[i][i] += sum_{j<n} |a[i][j]| (sequential sum; the 16-bit solves' matrix) */
+void   lu2o_diag_dominant(int n, double *a, long lda);
 void   lu2o_random_aug(int rows, int cols, long seed, double *a, long lda);
 int    lu2o_gepp_solve(int n, int nrhs, double *a, long lda, int *piv, double *x);
 

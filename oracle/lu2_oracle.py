@@ -49,6 +49,7 @@ def lib() -> ctypes.CDLL:
         L.lu2o_factor_solve.argtypes = [c_int, _dp, c_long, c_int, _ip, _dp]
         L.lu2o_cm_panel.argtypes = [c_int, _dp, c_long, c_int, _ip, c_int, c_int]
         L.lu2o_cm_panel.restype = c_int
+        L.lu2o_diag_dominant.argtypes = [c_int, _dp, c_long]
         L.lu2o_random_aug.argtypes = [c_int, c_int, c_long, _dp, c_long]
         L.lu2o_gepp_solve.argtypes = [c_int, c_int, _dp, c_long, _ip, _dp]
         L.lu2o_gepp_solve.restype = c_int
@@ -181,3 +182,10 @@ def hpl2_multi(a_aug: np.ndarray, x: np.ndarray) -> float:
     eps = 2.0 ** -53
     res = np.abs(r).max(axis=0) / (eps * (an * np.abs(x).max(axis=0) + np.abs(B).max(axis=0)) * n)
     return float(res.max())
+
+
+def diag_dominant(a: np.ndarray) -> np.ndarray:
+    """Copy of the augmented matrix with a[i][i] += sum_{j<n} |a[i][j]|."""
+    f = np.array(a, dtype=np.float64, copy=True, order="C")
+    lib().lu2o_diag_dominant(f.shape[0], f, f.shape[1])
+    return f
