@@ -212,16 +212,22 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const typename H16<FMT>:
 #pragma unroll
             for (int j = 0; j < 4; j++) bf[p][j] = *(const V8 *)&Bs[cur][p][wn * 64 + j * 16 + l16][lk * 8];
         }
+        // one product over all 16 accumulators before the next: consecutive
+        // matrix-core ops never wait on each other's accumulator
+        if (NP == 2) {   // small terms first
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = H16<FMT>::mfma(af[0][i], bf[NP - 1][j], acc[i][j]);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = H16<FMT>::mfma(af[NP - 1][i], bf[0][j], acc[i][j]);
+        }
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (NP == 2) {   // small terms first
-                    acc[i][j] = H16<FMT>::mfma(af[0][i], bf[NP - 1][j], acc[i][j]);
-                    acc[i][j] = H16<FMT>::mfma(af[NP - 1][i], bf[0][j], acc[i][j]);
-                }
-                acc[i][j] = H16<FMT>::mfma(af[0][i], bf[0][j], acc[i][j]);
-            }
+            for (int j = 0; j < 4; j++) acc[i][j] = H16<FMT>::mfma(af[0][i], bf[0][j], acc[i][j]);
         if (kt + 1 < nk) {
             store(cur ^ 1);
             __syncthreads();

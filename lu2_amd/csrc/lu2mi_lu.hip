@@ -541,13 +541,27 @@ __global__ void copy_col_kernel(const T *A, long lda, int n, int col, T *x)
 // ---------------------------------------------------------------------------
 // Back substitution on the unit-upper U (lu2.c:110-116) in one persistent
 // launch.  Rows are cut into 64-row blocks, solved bottom-up; block number
-// idx (0 = bottom) belongs to workgroup idx % G.  A workgroup accumulates
-// U[block rows][solved block] * x[solved block] for every block below it as
-// soon as the shared frontier (count of solved blocks) shows it solved, then
-// solves its 64x64 diagonal block in LDS with one wavefront, publishes x with
-// device-scope stores and advances the frontier.  x holds y on entry.
-// TF is the factor's element type, TX the vector's (FP64 for refinement).
+// idx (0 = bottom) belongs to workgroup idx % G (all G co-resident).  A
+// workgroup accumulates U[block rows][solved block] * x[solved block] for
+// every block below it as soon as the shared frontier (count of solved
+// blocks) shows it solved -- the A tile of the next block it will need is
+// loaded into registers before it waits, so a handoff costs only the x load
+// -- then solves its 64x64 diagonal block right-looking in one wavefront
+// (each finished x broadcast with readlane), publishes x with device-scope
+// stores and advances the frontier.  x holds y on entry.  TF is the factor's
+// element type, TX the vector's (FP64 for refinement).
 constexpr int BSB = 64;
+
+__device__ __forceinline__ double lane_bcast(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_bcast(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 
 template <typename TF, typename TX>
 __global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, long lda, int n, TX *x,
@@ -569,9 +583,26 @@ __global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, 
         }
         // wave w accumulates rows r0 + 16w .. r0 + 16w + 15, lane = column in the solved block
         TX acc[16];
+        TF pa[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) acc[i] = TX(0);
+        // A tile of solved block d for this thread: rows r0 + 16w + i, column c0(d) + lane
+        auto fetch = [&](int d, TF *v) {
+            const int col = (nblk - 1 - d) * BSB + lane;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int r = r0 + w * 16 + i;
+                v[i] = (r < r1 && col < n) ? A[(long)r * lda + col] : TF(0);
+            }
+        };
+        auto accumulate = [&](int d, const TF *v) {
+            const int col = (nblk - 1 - d) * BSB + lane;
+            const TX xv = (col < n) ? ld_buf(xr, (int)(sizeof(TX) * col), TX(0)) : TX(0);
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[i] += (TX)v[i] * xv;
+        };
         int done = 0;
+        if (idx > 0) fetch(0, pa);
         while (done < idx) {
             if (t == 0) {
                 unsigned long spins = 0;
@@ -592,19 +623,22 @@ __global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, 
             __syncthreads();
             const int F = min(s_front, idx);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            for (int d = done; d < F; d++) {
-                const int c0 = (nblk - 1 - d) * BSB;
-                const int col = c0 + lane;
-                if (col < n) {
-                    const TX xv = ld_buf(xr, (int)(sizeof(TX) * col), TX(0));
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        const int r = r0 + w * 16 + i;
-                        if (r < r1) acc[i] += (TX)A[(long)r * lda + col] * xv;
-                    }
-                }
+            accumulate(done, pa);            // the prefetched tile
+            int d = done + 1;
+            for (; d + 1 < F; d += 2) {      // blocks already solved: two tiles in flight
+                TF v0[16], v1[16];
+                fetch(d, v0);
+                fetch(d + 1, v1);
+                accumulate(d, v0);
+                accumulate(d + 1, v1);
+            }
+            if (d < F) {
+                TF v0[16];
+                fetch(d, v0);
+                accumulate(d, v0);
             }
             done = F;
+            if (done < idx) fetch(done, pa);   // ahead of the next wait
             __syncthreads();
         }
 #pragma unroll
@@ -616,12 +650,11 @@ __global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, 
         }
         __syncthreads();
         if (t < 64) {
+            // right-looking: x[rr] is final when step rr begins; rows above subtract U[t][rr] x[rr]
             TX xt = (t < nrb) ? x[r0 + t] - part[t] : TX(0);
-            for (int rr = nrb - 1; rr >= 0; rr--) {
-                TX v = (t > rr && t < nrb) ? (TX)Ud[rr][t] * xt : TX(0);
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (t == rr) xt -= v;
+            for (int rr = nrb - 1; rr > 0; rr--) {
+                const TX xk = lane_bcast(xt, rr);
+                if (t < rr) xt -= (TX)Ud[t][rr] * xk;
             }
             if (t < nrb) st_dev(x + r0 + t, xt);
             __builtin_amdgcn_s_waitcnt(0);
