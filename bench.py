@@ -1,7 +1,8 @@
-"""Benchmark: lu2 LU + solve on one MI355X (FP64 path).
+"""Benchmark: lu2 LU + solve on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 65536] [--nb 512]
-                    [--pivot gepp|compat] [--no-cpu]
+                    [--pivot gepp|compat] [--no-cpu] [--no-mixed] [--no-lowp]
+                    [--no-batch]
 
 A step is one full factor + solve of lu2's augmented n x (n+1) system
 (lu2.c:1122 lumcolumn: blocked recursive LU with partial pivoting, then the
@@ -12,14 +13,25 @@ timed region.  Each step is bracketed by synchronisations and timed; the
 value is 2n^3/3 flops (the metric's count, lu2.c:1304) over the mean step
 time.  The HPL-2 scaled residual of the last solution is checked (< 16).
 
-The roofline object describes the dominant kernel, the trailing-update
-DGEMM (A22 -= L21*U12, lu2.c:580): its algorithmic flops over the summed
-duration of its launches, from HIP events recorded by the library on the
-stream the kernel runs on, against the MI355X FP64 matrix peak.
+roofline: the dominant kernel, the trailing-update DGEMM (A22 -= L21*U12,
+lu2.c:580): its algorithmic flops over the summed duration of its launches,
+from HIP events recorded by the library on the stream the kernel runs on,
+against the MI355X FP64 matrix peak.
 
-mixed_precision: the same system solved by an FP32 factorization (FP32 MFMA
-trailing updates) and FP64 iterative refinement to the same residual bar,
-with its time-to-solution, iterations and error against the FP64 solution.
+mixed_precision: the same system solved by an LU of an FP32 copy whose
+trailing updates run on FP16 operands split hi + lo (FP16x2) or on FP32
+operands, then FP64 refinement (classical, GMRES-IR when it stalls) to the
+HPL-2 residual target; time-to-solution, iterations, error against the FP64
+solution, speed-up over the FP64 step.
+
+lowp_16bit: the 16-bit configuration -- lu2's matrix made diagonally
+dominant (a_ii += sum_j |a_ij|), n=98304, trailing updates on FP16 operands
+with per-block power-of-two scaling, FP64 refinement; error against an FP64
+GPU solve of the same system.
+
+batched: 4096 independent systems, n ~ U{32..512}, 1-4 right-hand sides,
+per-system drand48 streams, in one graph-captured persistent launch with the
+autotuned shape; FP64 and FP32-factor + FP64-refinement forms.
 
 cpu_baseline: the CPU oracle (a C restatement of lu2's algorithm, OpenMP) on
 a bounded sample, n=16384, on the host cores.
@@ -60,6 +72,90 @@ def cpu_baseline(n: int, nb: int, pivot: int):
                       f"seed 1, {dt:.2f} s wall, HPL-2 residual {res[6]:.3g}"}
 
 
+def lowp_config(ctx, n: int, nb: int) -> dict:
+    """16-bit trailing updates on lu2's matrix made diagonally dominant."""
+    import numpy as np
+
+    import lu2_amd as L
+
+    lda = n + 16
+    dA0 = ctx.alloc(8 * n * lda)
+    dx = ctx.alloc(8 * n)
+    ctx.randomsetmat(n, 1, dA0, lda)
+    ctx.make_diag_dominant(n, dA0, lda)
+    ctx.set_option(L.OPT_PIVMODE, L.PIV_GEPP)
+    res = {}
+    for name, prec in (("fp16", L.PREC_FP16), ("bf16", L.PREC_BF16)):
+        L._lib.lu2mi_synchronize(ctx.handle)
+        t0 = time.perf_counter()
+        info = ctx.solve_mixed(n, dA0, lda, nb, dx, prec, maxiter=30, target=1.0)
+        dt = time.perf_counter() - t0
+        res[name] = {"ms_per_solve": dt * 1e3, "value": 2.0 * n ** 3 / 3 / dt / 1e12, "unit": "TFLOP/s",
+                     "factor_ms": info["t_factor_ms"], "refine_ms": info["t_refine_ms"],
+                     "iterations": info["iterations"], "gmres_used": bool(info["gmres_used"]),
+                     "hpl2_residual_initial": info["resid_hpl2_initial"],
+                     "hpl2_residual": float(ctx.residual_dev(n, dA0, lda, dx)[6]),
+                     "x": dx.download(np.float64, n)}
+        log(f"lowp {name} n={n} nb={nb}: {dt:.3f} s iterations {info['iterations']} "
+            f"residual {res[name]['hpl2_residual']:.3g}")
+    # the FP64 GPU solve of the same system, for the error figures
+    dA = ctx.alloc(8 * n * lda)
+    dA.copy_from(dA0)
+    L._lib.lu2mi_synchronize(ctx.handle)
+    t0 = time.perf_counter()
+    ctx.lumcolumn_dev(n, dA, lda, 512, None, dx)
+    t64 = time.perf_counter() - t0
+    x64 = dx.download(np.float64, n)
+    for b in (dA, dA0, dx):
+        b.free()
+    for v in res.values():
+        v["err_vs_fp64"] = float(np.abs(v.pop("x") - x64).max() / np.abs(x64).max())
+        v["speedup_vs_fp64"] = t64 * 1e3 / v["ms_per_solve"]
+    log(f"lowp reference FP64 solve n={n}: {t64:.3f} s")
+    return {"workload": f"lu2 matrix + diag(row abs-sum), n={n}, nb={nb}, pivot=gepp, seed 1",
+            "fp64_ms": t64 * 1e3, **res}
+
+
+def batched_config(ctx, seed: int) -> dict:
+    """4096 systems, n ~ U{32..512}, 1-4 right-hand sides, tuned launch shape."""
+    import numpy as np
+
+    import lu2_amd as L
+
+    rng = np.random.default_rng(2024)
+    nsys = 4096
+    n = rng.integers(32, 513, nsys)
+    nrhs = rng.integers(1, 5, nsys)
+    flops = float((2.0 * n.astype(float) ** 3 / 3 + 2.0 * n.astype(float) ** 2 * nrhs).sum())
+    out = {"workload": f"{nsys} systems, n ~ U{{32..512}}, nrhs ~ U{{1..4}}, per-system drand48 "
+                       f"(seed base {seed} + system id), one graph-captured persistent launch",
+           "flops_per_batch": flops}
+    dA = dX = None
+    for name, prec in (("fp64", L.PREC_FP64), ("fp32_ir", L.PREC_FP32)):
+        b = L.Batch(ctx, n, nrhs, prec=prec, maxiter=10, target=1.0)
+        if dA is None:
+            dA = ctx.alloc(8 * b.a_elems)
+            dX = ctx.alloc(8 * b.x_elems)
+            b.generate(dA, seed)
+        cfg = b.tune(dA, dX, reps=2)
+        infos = [b.solve(dA, dX) for _ in range(4)]       # first replay captures the graph
+        ms = sum(i["t_ms"] for i in infos[1:]) / 3
+        info = infos[-1]
+        _, iters, _ = b.results()
+        out[name] = {"ms_per_batch": ms, "systems_per_s": nsys / ms * 1e3, "value": flops / ms / 1e9,
+                     "unit": "TFLOP/s", "frac_of_fp64_peak": flops / ms / 1e9 / FP64_MFMA_PEAK_TFLOPS,
+                     "max_hpl2_residual": info["max_resid_hpl2"], "failed": info["n_failed"],
+                     "max_iterations": int(iters.max()), "mean_iterations": float(iters.mean()),
+                     "tuned": {"threads_per_system": cfg[0], "panel_width_small": cfg[1],
+                               "panel_width_large": cfg[2], "split_n": cfg[3]}}
+        log(f"batched {name}: {ms:.2f} ms  {nsys / ms * 1e3:.0f} systems/s  max residual "
+            f"{info['max_resid_hpl2']:.3g}  tuned {cfg}")
+        b.close()
+    dA.free()
+    dX.free()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +169,11 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=1)
+    ap.add_argument("--mixed-nb", type=int, default=1024)
+    ap.add_argument("--no-lowp", action="store_true")
+    ap.add_argument("--lowp-n", type=int, default=98304)
+    ap.add_argument("--lowp-nb", type=int, default=2048)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -127,30 +228,38 @@ def main() -> int:
 
     mixed = None
     if not args.no_mixed:
-        # FP32 factorization + FP64 refinement on the same matrix (dA0 is read only)
+        # FP32-copy LU with FP16x2 / FP32 trailing updates + FP64 refinement on
+        # the same matrix (dA0 is read only)
         x64 = dx.download(np.float64, n)
         dxm = ctx.alloc(8 * n)
+        mixed = {}
         try:
-            tms = []
-            for w in range(args.warmup + args.mixed_steps):
-                L._lib.lu2mi_synchronize(ctx.handle)
-                t0 = time.perf_counter()
-                info = ctx.solve_mixed(n, dA0, lda, nb, dxm, L.PREC_FP32, maxiter=30, target=1.0)
-                dt = time.perf_counter() - t0
-                if w >= args.warmup:
-                    tms.append(dt)
-                log(f"mixed {'warmup' if w < args.warmup else 'step'}: {dt:.3f} s  iterations "
-                    f"{info['iterations']} (gmres {info['gmres_used']}) residual {info['resid_hpl2']:.3g}")
-            xm = dxm.download(np.float64, n)
-            rm = ctx.residual_dev(n, dA0, lda, dxm)
-            tm = sum(tms) / len(tms)
-            mixed = {"precision": "fp32 factors, fp32 MFMA updates, fp64 refinement",
-                     "value": flops / tm / 1e12, "unit": "TFLOP/s", "ms_per_solve": tm * 1e3,
-                     "factor_ms": info["t_factor_ms"], "refine_ms": info["t_refine_ms"],
-                     "iterations": info["iterations"], "gmres_used": bool(info["gmres_used"]),
-                     "hpl2_residual_initial": info["resid_hpl2_initial"], "hpl2_residual": float(rm[6]),
-                     "err_vs_fp64": float(np.abs(xm - x64).max() / np.abs(x64).max()),
-                     "speedup_vs_fp64": mean / tm}
+            for name, prec, mnb in (("fp16x2", L.PREC_FP16X2, args.mixed_nb), ("fp32", L.PREC_FP32, nb)):
+                tms = []
+                for w in range(args.warmup + args.mixed_steps):
+                    L._lib.lu2mi_synchronize(ctx.handle)
+                    t0 = time.perf_counter()
+                    info = ctx.solve_mixed(n, dA0, lda, mnb, dxm, prec, maxiter=30, target=1.0)
+                    dt = time.perf_counter() - t0
+                    if w >= args.warmup:
+                        tms.append(dt)
+                    log(f"mixed {name} {'warmup' if w < args.warmup else 'step'}: {dt:.3f} s  iterations "
+                        f"{info['iterations']} (gmres {info['gmres_used']}) residual {info['resid_hpl2']:.3g}")
+                xm = dxm.download(np.float64, n)
+                rm = ctx.residual_dev(n, dA0, lda, dxm)
+                tm = sum(tms) / len(tms)
+                mixed[name] = {
+                    "precision": ("FP32 factors; trailing updates on FP16 operands split hi+lo "
+                                  "(hi*hi+hi*lo+lo*hi), per-block power-of-two scaling, FP32 accumulation"
+                                  if prec == L.PREC_FP16X2 else
+                                  "FP32 factors; FP32 MFMA trailing updates, FP32 accumulation"),
+                    "nb": mnb, "value": flops / tm / 1e12, "unit": "TFLOP/s", "ms_per_solve": tm * 1e3,
+                    "factor_ms": info["t_factor_ms"], "refine_ms": info["t_refine_ms"],
+                    "iterations": info["iterations"], "gmres_used": bool(info["gmres_used"]),
+                    "gmres_inner": info["gmres_inner"],
+                    "hpl2_residual_initial": info["resid_hpl2_initial"], "hpl2_residual": float(rm[6]),
+                    "err_vs_fp64": float(np.abs(xm - x64).max() / np.abs(x64).max()),
+                    "speedup_vs_fp64": mean / tm}
         finally:
             dxm.free()
 
@@ -176,12 +285,18 @@ def main() -> int:
         "hpl2_residual": float(res[6]),
         "roofline": {
             "bound": "mfma",
-            "kernel": "gemm_f64_kernel (trailing update A22 -= L21*U12)",
+            "kernel": "gemm_kernel<double, true, 1, 1> (trailing update A22 -= L21*U12)",
             "achieved": achieved,
             "peak": FP64_MFMA_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved / FP64_MFMA_PEAK_TFLOPS) if achieved else None,
-            "traffic": None,
+            # HBM bytes of one trailing update of the factorization's M=N=32768, K=512 shape,
+            # from FETCH_SIZE and WRITE_SIZE counter runs scaled on a 1 GiB copy
+            # (profiles/trailing_dgemm_m32768_k512_pmc.md); the algorithmic bytes of
+            # that launch are 17.45e9
+            "traffic": 32.37e9,
+            "traffic_launch": "M=N=32768 K=512",
+            "traffic_algorithmic": 17.45e9,
             "launches_per_step": gemm_launch // max(1, args.steps),
             "avg_launch_ms": gemm_ms / max(1, gemm_launch),
         },
@@ -190,6 +305,10 @@ def main() -> int:
         out["mixed_precision"] = mixed
     for b in (dA0, dA, dx, dp):
         b.free()
+    if not args.no_lowp:
+        out["lowp_16bit"] = lowp_config(ctx, args.lowp_n, args.lowp_nb)
+    if not args.no_batch:
+        out["batched"] = batched_config(ctx, args.seed)
     ctx.close()
     if not args.no_cpu:
         log(f"cpu baseline: oracle n={args.cpu_n}")
