@@ -1,6 +1,6 @@
 """Benchmark: lu2 LU + solve on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 65536] [--nb 512]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 65536] [--nb 2048]
                     [--pivot gepp|compat] [--no-cpu] [--no-mixed] [--no-lowp]
                     [--no-batch]
 
@@ -162,7 +162,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=65536)
-    ap.add_argument("--nb", type=int, default=512)
+    ap.add_argument("--nb", type=int, default=2048)
     ap.add_argument("--pivot", choices=["gepp", "compat"], default="gepp")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -174,6 +174,8 @@ def main() -> int:
     ap.add_argument("--lowp-n", type=int, default=98304)
     ap.add_argument("--lowp-nb", type=int, default=2048)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--lookahead", type=int, default=0, choices=(0, 1),
+                    help="factor the next panel on a second stream during the trailing update")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -192,6 +194,7 @@ def main() -> int:
     ncu, arch = ctx.device_info()
     log(f"device {arch} CUs={ncu}; n={n} nb={nb} pivot={args.pivot}")
     ctx.set_option(L.OPT_PIVMODE, piv_mode)
+    ctx.set_option(L.OPT_LOOKAHEAD, args.lookahead)
     nbytes = 8 * n * lda
     dA0 = ctx.alloc(nbytes)
     dA = ctx.alloc(nbytes)
@@ -234,7 +237,7 @@ def main() -> int:
         dxm = ctx.alloc(8 * n)
         mixed = {}
         try:
-            for name, prec, mnb in (("fp16x2", L.PREC_FP16X2, args.mixed_nb), ("fp32", L.PREC_FP32, nb)):
+            for name, prec, mnb in (("fp16x2", L.PREC_FP16X2, args.mixed_nb), ("fp32", L.PREC_FP32, args.mixed_nb)):
                 tms = []
                 for w in range(args.warmup + args.mixed_steps):
                     L._lib.lu2mi_synchronize(ctx.handle)
@@ -279,7 +282,7 @@ def main() -> int:
         "data": "synthetic: lu2 drand48 matrix (lu2.c:70-85) generated on the device",
         "config": {"workload": f"lu2 FP64 LU with partial pivoting + solve, n={n}, nb={nb}, "
                                f"pivot={args.pivot}, 1 RHS, seed {args.seed}",
-                   "n": n, "nb": nb, "pivot": args.pivot},
+                   "n": n, "nb": nb, "pivot": args.pivot, "lookahead": args.lookahead},
         "tflops_2n3_3_plus_2n2": (flops + 2.0 * n * n) / mean / 1e12,
         "frac_of_fp64_peak": flops / mean / 1e12 / FP64_MFMA_PEAK_TFLOPS,
         "hpl2_residual": float(res[6]),

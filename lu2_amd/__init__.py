@@ -36,6 +36,7 @@ LOWP_FP16X2 = 3
 OPT_PIVMODE = 1
 OPT_LEAF_W = 2
 OPT_TIMING = 3
+OPT_LOOKAHEAD = 4
 BATCH_EAGER = 1
 BATCH_NMAX = 512
 BATCH_RHSMAX = 8

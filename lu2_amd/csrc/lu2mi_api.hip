@@ -53,6 +53,11 @@ extern "C" int lu2mi_init(int device, lu2mi_ctx **out)
     LU2MI_CHECK(hipGetDeviceProperties(&prop, device));
     ctx->num_cu = prop.multiProcessorCount;
     LU2MI_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // the panel stream of the lookahead schedule (default priority, like the main one)
+    LU2MI_CHECK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+    LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_next, hipEventDisableTiming));
+    LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming));
+    ctx->lookahead = 0;
     LU2MI_CHECK(hipMalloc(&ctx->d_bar, 256));
     LU2MI_CHECK(hipMemset(ctx->d_bar, 0, 256));
     LU2MI_CHECK(hipMalloc(&ctx->d_err, 256));
@@ -84,6 +89,10 @@ extern "C" int lu2mi_finalize(lu2mi_ctx *ctx)
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     free_timer(ctx->tgemm);
+    (void)hipStreamSynchronize(ctx->stream2);
+    (void)hipEventDestroy(ctx->ev_next);
+    (void)hipEventDestroy(ctx->ev_panel);
+    (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     free(ctx);
     return LU2MI_OK;
@@ -100,6 +109,9 @@ extern "C" int lu2mi_set_option(lu2mi_ctx *ctx, int opt, int value)
     case LU2MI_OPT_LEAF_W:
         if (value != 16 && value != 32) return LU2MI_ERR_ARG;
         ctx->leaf_w = value;
+        return LU2MI_OK;
+    case LU2MI_OPT_LOOKAHEAD:
+        ctx->lookahead = value ? 1 : 0;
         return LU2MI_OK;
     case LU2MI_OPT_TIMING:
         ctx->timing_on = value ? 1 : 0;

@@ -74,6 +74,9 @@ struct lu2mi_ctx {
     int pivmode;
     int leaf_w;
     int timing_on;
+    int lookahead;         // factor panel i+1 on stream2 during block i's trailing update
+    hipStream_t stream2;
+    hipEvent_t ev_next, ev_panel;
     KernelTimer tgemm;
     double t_last_ms[8];
     // mixed precision: FP32 working copy, kept per-block factors, refinement vectors

@@ -24,8 +24,8 @@ typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 
 namespace {
 constexpr int TB = 128;    // scaling block = output tile edge
-constexpr int BKH = 32;    // K per LDS stage (16-bit elements)
-constexpr int PADH = 8;    // LDS row pad (elements): rows 80 bytes apart
+constexpr int BKH = 64;    // K per LDS stage (16-bit elements): two 32-deep matrix-core steps
+constexpr int PADH = 8;    // LDS row pad (elements): rows 144 bytes apart, conflict-free fragment reads
 
 template <int FMT> struct H16;
 template <> struct H16<LU2MI_LOWP_FP16> {
@@ -151,6 +151,15 @@ __global__ __launch_bounds__(256) void scale_cols16_kernel(const float *__restri
 // Lane l supplies A[l%16][8(l/16)+e] and B[8(l/16)+e][l%16]: the same K
 // assignment in both operands, so the product is the GEMM's whatever order
 // the instruction sums in; the result lands at row 4(l/16)+r, column l%16.
+// C[M x N] -= 2^-(sa[tm]+sb[tn]) * A16 B16t^T; A16 and B16t padded to whole
+// 128-row tiles.  256 threads, 2x2 waves of 64x64, 16x16x32 matrix-core
+// blocks.  K is staged 64 at a time through LDS; the next stage's global
+// loads are in flight in registers while the current stage computes.  One
+// operand plane keeps two LDS buffers (one barrier per stage); the split
+// format keeps one (two barriers per stage) so that two workgroups still fit
+// a CU.  Lane l supplies A[l%16][8(l/16)+e] and B[8(l/16)+e][l%16]: the same K
+// assignment in both operands, so the product is the GEMM's whatever order
+// the instruction sums in; the result lands at row 4(l/16)+r, column l%16.
 template <int FMT>
 __global__ __launch_bounds__(256, 2) void gemm16_kernel(const typename H16<FMT>::t *__restrict__ A16,
                                                         const typename H16<FMT>::t *__restrict__ B16t, int M, int N,
@@ -160,9 +169,12 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const typename H16<FMT>:
     typedef typename H16<FMT>::t H;
     typedef typename H16<FMT>::v8 V8;
     constexpr int NP = H16<FMT>::NP;
+    constexpr int NBUF = (NP == 1) ? 2 : 1;
+    constexpr int CPR = BKH / 8;              // 16-byte chunks per staged row
+    constexpr int LPT = TB * CPR / 256;       // chunk loads per thread per operand plane
     using namespace lu2mi_tiles;
-    __shared__ __align__(16) H As[2][NP][TB][BKH + PADH];
-    __shared__ __align__(16) H Bs[2][NP][TB][BKH + PADH];
+    __shared__ __align__(16) H As[NBUF][NP][TB][BKH + PADH];
+    __shared__ __align__(16) H Bs[NBUF][NP][TB][BKH + PADH];
     const int tiles_m = (M + TB - 1) / TB, tiles_n = (N + TB - 1) / TB;
     int tm, tn;
     tile_coords(xcd_remap(blockIdx.x, tiles_m * tiles_n), tiles_m, tiles_n, tm, tn);
@@ -170,13 +182,13 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const typename H16<FMT>:
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w >> 1, wn = w & 1, l16 = lane & 15, lk = lane >> 4;
 
-    V8 ra[NP][2], rb[NP][2];
+    V8 ra[NP][LPT], rb[NP][LPT];
     auto load = [&](int k0) {
 #pragma unroll
         for (int p = 0; p < NP; p++)
 #pragma unroll
-            for (int it = 0; it < 2; it++) {
-                const int q = t + it * 256, row = q >> 2, part = (q & 3) * 8;
+            for (int it = 0; it < LPT; it++) {
+                const int q = t + it * 256, row = q / CPR, part = (q % CPR) * 8;
                 ra[p][it] = *(const V8 *)(A16 + p * pa + (long)(m0 + row) * K + k0 + part);
                 rb[p][it] = *(const V8 *)(B16t + p * pb + (long)(n0 + row) * K + k0 + part);
             }
@@ -185,8 +197,8 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const typename H16<FMT>:
 #pragma unroll
         for (int p = 0; p < NP; p++)
 #pragma unroll
-            for (int it = 0; it < 2; it++) {
-                const int q = t + it * 256, row = q >> 2, part = (q & 3) * 8;
+            for (int it = 0; it < LPT; it++) {
+                const int q = t + it * 256, row = q / CPR, part = (q % CPR) * 8;
                 *(V8 *)&As[buf][p][row][part] = ra[p][it];
                 *(V8 *)&Bs[buf][p][row][part] = rb[p][it];
             }
@@ -202,34 +214,38 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const typename H16<FMT>:
     store(0);
     __syncthreads();
     for (int kt = 0; kt < nk; kt++) {
-        const int cur = kt & 1;
+        const int cur = (NBUF == 2) ? (kt & 1) : 0;
         if (kt + 1 < nk) load((kt + 1) * BKH);
-        V8 af[NP][4], bf[NP][4];
 #pragma unroll
-        for (int p = 0; p < NP; p++) {
+        for (int ks = 0; ks < BKH; ks += 32) {
+            V8 af[NP][4], bf[NP][4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) af[p][i] = *(const V8 *)&As[cur][p][wm * 64 + i * 16 + l16][lk * 8];
+            for (int p = 0; p < NP; p++) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) bf[p][j] = *(const V8 *)&Bs[cur][p][wn * 64 + j * 16 + l16][lk * 8];
-        }
-        // one product over all 16 accumulators before the next: consecutive
-        // matrix-core ops never wait on each other's accumulator
-        if (NP == 2) {   // small terms first
+                for (int i = 0; i < 4; i++) af[p][i] = *(const V8 *)&As[cur][p][wm * 64 + i * 16 + l16][ks + lk * 8];
+#pragma unroll
+                for (int j = 0; j < 4; j++) bf[p][j] = *(const V8 *)&Bs[cur][p][wn * 64 + j * 16 + l16][ks + lk * 8];
+            }
+            // one product over all 16 accumulators before the next: consecutive
+            // matrix-core ops never wait on each other's accumulator
+            if (NP == 2) {   // small terms first
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[i][j] = H16<FMT>::mfma(af[0][i], bf[NP - 1][j], acc[i][j]);
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[i][j] = H16<FMT>::mfma(af[NP - 1][i], bf[0][j], acc[i][j]);
+            }
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = H16<FMT>::mfma(af[0][i], bf[NP - 1][j], acc[i][j]);
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = H16<FMT>::mfma(af[NP - 1][i], bf[0][j], acc[i][j]);
+                for (int j = 0; j < 4; j++) acc[i][j] = H16<FMT>::mfma(af[0][i], bf[0][j], acc[i][j]);
         }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] = H16<FMT>::mfma(af[0][i], bf[0][j], acc[i][j]);
         if (kt + 1 < nk) {
-            store(cur ^ 1);
+            if (NBUF == 1) __syncthreads();   // every wave is done reading the single buffer
+            store((NBUF == 2) ? (cur ^ 1) : 0);
             __syncthreads();
         }
     }
@@ -249,8 +265,6 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const typename H16<FMT>:
             }
 }
 
-// C -= L U with 16-bit operands; L: M x K (ldl), U: K x N (ldu), C: M x N (ldc),
-// all FP32 row-major on the context's device.  K a multiple of 64.
 namespace {
 template <int FMT>
 void launch16(lu2mi_ctx *ctx, int M, int N, int K, const float *L, long ldl, const float *U, long ldu, float *C,

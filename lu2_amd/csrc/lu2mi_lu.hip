@@ -813,8 +813,9 @@ static int backsub(lu2mi_ctx *ctx, const TF *A, long lda, int n, TX *x)
     const int nblk = (n + BSB - 1) / BSB;
     int per_cu = 0;
     LU2MI_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)backsub_kernel<TF, TX>, 256, 0));
-    int G = ctx->num_cu < nblk ? ctx->num_cu : nblk;
-    if (G > per_cu * ctx->num_cu) G = per_cu * ctx->num_cu;
+    // every workgroup the device keeps co-resident (2 per CU for these
+    // kernels): more workgroups mean shorter backlogs of solved tiles each
+    int G = per_cu * ctx->num_cu < nblk ? per_cu * ctx->num_cu : nblk;
     if (G < 1) G = 1;
     LU2MI_CHECK(hipMemsetAsync(ctx->d_bar, 0, sizeof(unsigned long long), ctx->stream));
     hipLaunchKernelGGL((backsub_kernel<TF, TX>), dim3(G), dim3(256), 0, ctx->stream, A, lda, n, x, ctx->d_bar,
@@ -940,21 +941,68 @@ struct Factor {
         return LU2MI_OK;
     }
 
+    // scratch pointers of block i (per-block copies when the factors are kept)
+    void set_block(int i)
+    {
+        if (linv_all) {
+            linv = linv_all + (long)(i / nb) * nb * nb;
+            pairs = pairs_all + (long)(i / nb) * 4 * nb;
+        }
+    }
+
+    int start_panel(int i)
+    {
+        set_block(i);
+        hipLaunchKernelGGL(fill_kernel<T>, dim3(256), dim3(256), 0, ctx->stream, linv, (long)nb * nb, T(0));
+        LU2MI_CHECK(hipGetLastError());
+        return panel(i, i, nb);
+    }
+
+    // A22[:, c0 .. c0+N) -= L21 * U12[:, c0 .. c0+N) for block i (lu2.c:580), timed
+    int trailing(int i, int c0, int N)
+    {
+        const int M = n - i - nb;
+        KernelTimer &kt = ctx->tgemm;
+        const bool tm = ctx->timing_on && kt.used < kt.cap;
+        if (tm) LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used], ctx->stream));
+        if (sizeof(T) == 4 && ctx->lowp)   // operands rounded to FP16/BF16, FP32 accumulation
+            LU2MI_TRY(lu2mi_update16(ctx, ctx->lowp, M, N, nb, (const float *)(A + (long)(i + nb) * lda + i), lda,
+                                     (const float *)(A + (long)i * lda + c0), lda,
+                                     (float *)(A + (long)(i + nb) * lda + c0), lda));
+        else
+            LU2MI_TRY(gemm_call<T>(ctx, M, N, nb, -1.0, A + (long)(i + nb) * lda + i, lda, A + (long)i * lda + c0,
+                                   lda, 1.0, A + (long)(i + nb) * lda + c0, lda, 0, 1));
+        if (tm) {
+            LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used + 1], ctx->stream));
+            kt.flops[kt.used] = 2.0 * M * (double)N * nb;
+            kt.used++;
+        }
+        return LU2MI_OK;
+    }
+
+    // Block loop of lu2.c:1122-1144.  With lookahead, block i's trailing
+    // update is split: the next panel's nb columns first, then the rest; the
+    // next panel is factored on the second stream as soon as its columns are
+    // up to date, while the rest of the update runs.  The two touch disjoint
+    // columns, and block i's pairs, staging rows and Linv are consumed before
+    // the split, so the panel may reuse them.
     int run(int *dpiv_out, T *dx)
     {
+        const bool la = ctx->lookahead != 0;
+        hipStream_t s1 = ctx->stream;
+        bool panel_pending = false;   // panel(i) issued on the second stream
+        LU2MI_TRY(start_panel(0));
         for (int i = 0; i < n; i += nb) {
-            if (linv_all) {
-                linv = linv_all + (long)(i / nb) * nb * nb;
-                pairs = pairs_all + (long)(i / nb) * 4 * nb;
+            if (panel_pending) {
+                LU2MI_CHECK(hipStreamWaitEvent(s1, ctx->ev_panel, 0));
+                panel_pending = false;
             }
-            hipLaunchKernelGGL(fill_kernel<T>, dim3(256), dim3(256), 0, ctx->stream, linv, (long)nb * nb, T(0));
-            LU2MI_CHECK(hipGetLastError());
-            LU2MI_TRY(panel(i, i, nb));
+            set_block(i);
             const int ncols = n + 1 - (i + nb);
             // block-level row permutation over [i+nb, n], 1/d scaling of the
             // block rows, block rows staged in w (lu2.c:502-509 awork copy)
-            hipLaunchKernelGGL(block_pairs_kernel, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream,
-                               ctx->d_piv, ctx->d_orig, i, nb, pairs, pairs + 2 * nb);
+            hipLaunchKernelGGL(block_pairs_kernel, dim3((nb + 255) / 256), dim3(256), 0, s1, ctx->d_piv,
+                               ctx->d_orig, i, nb, pairs, pairs + 2 * nb);
             LU2MI_CHECK(hipGetLastError());
             const long ldw = (ncols + 3) & ~3L;   // 16-byte rows for FP64 and FP32
             LU2MI_TRY(rowperm(pairs, pairs + 2 * nb, 2 * nb, i + nb, n + 1, 0, 0, n + 1, i, i + nb, wbuf, ldw, i,
@@ -962,27 +1010,25 @@ struct Factor {
             // U12 = inv(D^-1 L11) * staged rows   (lu2.c:511-512)
             LU2MI_TRY(gemm_call<T>(ctx, nb, ncols, nb, 1.0, linv, nb, wbuf, ldw, 0.0, A + (long)i * lda + i + nb,
                                    lda, 1));
-            // A22 -= L21 * U12   (lu2.c:580)
             const int M = n - i - nb;
-            if (M > 0) {
-                KernelTimer &kt = ctx->tgemm;
-                const bool tm = ctx->timing_on && kt.used < kt.cap;
-                if (tm) LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used], ctx->stream));
-                if (sizeof(T) == 4 && ctx->lowp)   // operands rounded to FP16/BF16, FP32 accumulation
-                    LU2MI_TRY(lu2mi_update16(ctx, ctx->lowp, M, ncols, nb, (const float *)(A + (long)(i + nb) * lda + i),
-                                             lda, (const float *)(A + (long)i * lda + i + nb), lda,
-                                             (float *)(A + (long)(i + nb) * lda + i + nb), lda));
-                else
-                    LU2MI_TRY(gemm_call<T>(ctx, M, ncols, nb, -1.0, A + (long)(i + nb) * lda + i, lda,
-                                           A + (long)i * lda + i + nb, lda, 1.0, A + (long)(i + nb) * lda + i + nb,
-                                           lda, 0, 1));
-                if (tm) {
-                    LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used + 1], ctx->stream));
-                    kt.flops[kt.used] = 2.0 * M * (double)ncols * nb;
-                    kt.used++;
-                }
+            if (M <= 0) continue;
+            if (la) {
+                LU2MI_TRY(trailing(i, i + nb, nb));                    // the next panel's columns
+                LU2MI_CHECK(hipEventRecord(ctx->ev_next, s1));
+                LU2MI_CHECK(hipStreamWaitEvent(ctx->stream2, ctx->ev_next, 0));
+                ctx->stream = ctx->stream2;                           // panel(i+nb) on the second stream
+                const int prc = start_panel(i + nb);
+                ctx->stream = s1;
+                LU2MI_TRY(prc);
+                LU2MI_CHECK(hipEventRecord(ctx->ev_panel, ctx->stream2));
+                panel_pending = true;
+                LU2MI_TRY(trailing(i, i + 2 * nb, ncols - nb));       // the rest, b included
+            } else {
+                LU2MI_TRY(trailing(i, i + nb, ncols));
+                LU2MI_TRY(start_panel(i + nb));
             }
         }
+        if (panel_pending) LU2MI_CHECK(hipStreamWaitEvent(s1, ctx->ev_panel, 0));
         // x = column n, then unit-upper back substitution
         hipLaunchKernelGGL(copy_col_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A, lda, n, n, dx);
         LU2MI_CHECK(hipGetLastError());

@@ -44,16 +44,14 @@ __global__ __launch_bounds__(256) void x0_b_kernel(const float *xf, const double
     }
 }
 
-// One block of the forward solve: r[dst] = r[src] over the block's row
-// pairs (all reads first), rows of the block scaled by 1/d, then
-// r_block = Linv * r_block.  One workgroup.
+// One block of the forward solve, first half: r[dst] = r[src] over the
+// block's row pairs (all reads first); the block's own rows, scaled by 1/d,
+// go to rb (the input of the block's Linv product).  One workgroup.
 template <typename TF>
-__global__ __launch_bounds__(1024) void fwd_block_kernel(double *r, int i, int nb, const int *pairs, const TF *invd,
-                                                         const TF *linv)
+__global__ __launch_bounds__(1024) void fwd_perm_kernel(double *r, int i, int nb, const int *pairs, const TF *invd,
+                                                        double *rb)
 {
-    extern __shared__ double sh[];
-    double *tmp = sh;            // 2*nb gathered values
-    double *rb = sh + 2 * nb;    // nb scaled block values
+    extern __shared__ double tmp[];   // 2*nb gathered values
     const int t = threadIdx.x;
     const int *dst = pairs, *src = pairs + 2 * nb;
     for (int p = t; p < 2 * nb; p += blockDim.x) tmp[p] = r[src[p]];
@@ -63,13 +61,24 @@ __global__ __launch_bounds__(1024) void fwd_block_kernel(double *r, int i, int n
         if (d < i || d >= i + nb) r[d] = tmp[p];   // rows outside the block
         else rb[d - i] = tmp[p] * (double)invd[d];
     }
+}
+
+// Second half: r[i+k] = sum_{m<=k} Linv[k][m] rb[m] (Linv unit lower); one
+// wavefront per row, spread over the whole device.
+template <typename TF>
+__global__ __launch_bounds__(256) void fwd_linv_kernel(double *r, int i, int nb, const TF *linv, const double *rb)
+{
+    extern __shared__ double y[];
+    for (int k = threadIdx.x; k < nb; k += 256) y[k] = rb[k];
     __syncthreads();
-    for (int k = t; k < nb; k += blockDim.x) {
-        const TF *row = linv + (long)k * nb;
-        double s = 0.0;
-        for (int m = 0; m <= k; m++) s = fma((double)row[m], rb[m], s);
-        r[i + k] = s;
-    }
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= nb) return;
+    const TF *row = linv + (long)k * nb;
+    double s = 0.0;
+    for (int m = lane; m <= k; m += 64) s = fma((double)row[m], y[m], s);
+    s = wave_sum(s);
+    if (lane == 0) r[i + k] = s;
 }
 
 // r[j] -= sum_k A[j][i+k] * r[i+k] for rows j in [i+nb, n); one wavefront per row
@@ -185,6 +194,7 @@ struct Mixed {
     double *vec;         // work vectors
     double *part;        // reduction partials (host-readable after sync)
     double *coef;        // coefficients for combine_kernel
+    double *scratch;     // nb values: a block's permuted, scaled right-hand side
     double anorm = 0, bnorm = 0;
     hipStream_t s() const { return ctx->stream; }
 
@@ -193,12 +203,14 @@ struct Mixed {
     // r <- (LU)^-1 r with the FP32 factors, FP64 vector arithmetic
     int lu_solve(double *r)
     {
-        const size_t shf = sizeof(double) * 3 * nb;
+        double *rb = scratch;   // nb values
         for (int i = 0; i < n; i += nb) {
             const int b = i / nb;
-            hipLaunchKernelGGL((fwd_block_kernel<float>), dim3(1), dim3(nb < 1024 ? nb : 1024), shf, s(), r, i, nb,
-                               pairs_all + (long)b * 4 * nb, (const float *)ctx->d_invd_f,
-                               linv_all + (long)b * nb * nb);
+            hipLaunchKernelGGL((fwd_perm_kernel<float>), dim3(1), dim3(nb < 1024 ? nb : 1024),
+                               sizeof(double) * 2 * nb, s(), r, i, nb, pairs_all + (long)b * 4 * nb,
+                               (const float *)ctx->d_invd_f, rb);
+            hipLaunchKernelGGL((fwd_linv_kernel<float>), dim3((nb + 3) / 4), dim3(256), sizeof(double) * nb, s(),
+                               r, i, nb, (const float *)(linv_all + (long)b * nb * nb), (const double *)rb);
             LU2MI_CHECK(hipGetLastError());
             const int rows = n - i - nb;
             if (rows > 0) {
@@ -377,6 +389,7 @@ extern "C" int lu2mi_solve_mixed(lu2mi_ctx *ctx, int n, const double *dA, long l
     M.vec = (double *)ctx->m_vec;
     M.part = M.vec + (size_t)(gm + 6) * n;
     M.coef = M.part + npart;
+    M.scratch = M.V(gm + 4);   // free once the initial norms are taken
     double *r = M.V(gm + 2), *d = M.V(gm + 3);
 
     auto t0 = std::chrono::steady_clock::now();

@@ -216,3 +216,26 @@ def test_lu_n32768_residual(ctx):
             d.free()
     assert res[6] < 16.0
     assert np.all(piv >= np.arange(n)) and np.all(piv < n)
+
+
+@pytest.mark.parametrize("n,nb,mode", [(4096, 256, L.PIV_GEPP), (2048, 512, L.PIV_COMPAT), (3072, 128, L.PIV_GEPP)])
+def test_lookahead_schedule_bit_identical(ctx, n, nb, mode):
+    """Factoring panel i+1 on the second stream during block i's split
+    trailing update performs the same arithmetic per element: factors,
+    pivots and x are bit-identical to the serial schedule."""
+    lda = n + 16
+    outs = []
+    for la in (0, 1):
+        dA = ctx.alloc(8 * n * lda)
+        dx = ctx.alloc(8 * n)
+        dp = ctx.alloc(4 * n)
+        ctx.randomsetmat(n, 3, dA, lda)
+        ctx.set_option(L.OPT_PIVMODE, mode)
+        ctx.set_option(L.OPT_LOOKAHEAD, la)
+        ctx.lumcolumn_dev(n, dA, lda, nb, dp, dx)
+        outs.append((dA.download(np.float64, n * lda), dx.download(np.float64, n), dp.download(np.int32, n)))
+        for d in (dA, dx, dp):
+            d.free()
+    ctx.set_option(L.OPT_LOOKAHEAD, 0)
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
