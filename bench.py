@@ -293,13 +293,13 @@ def main() -> int:
             "peak": FP64_MFMA_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved / FP64_MFMA_PEAK_TFLOPS) if achieved else None,
-            # HBM bytes of one trailing update of the factorization's M=N=32768, K=512 shape,
+            # bytes of one trailing update of the factorization's M=N=32768, K=2048 shape,
             # from FETCH_SIZE and WRITE_SIZE counter runs scaled on a 1 GiB copy
-            # (profiles/trailing_dgemm_m32768_k512_pmc.md); the algorithmic bytes of
-            # that launch are 17.45e9
-            "traffic": 32.37e9,
-            "traffic_launch": "M=N=32768 K=512",
-            "traffic_algorithmic": 17.45e9,
+            # (profiles/trailing_dgemm_m32768_k2048_pmc.md); the algorithmic bytes of
+            # that launch are 18.25e9
+            "traffic": 68.70e9,
+            "traffic_launch": "M=N=32768 K=2048",
+            "traffic_algorithmic": 18.25e9,
             "launches_per_step": gemm_launch // max(1, args.steps),
             "avg_launch_ms": gemm_ms / max(1, gemm_launch),
         },
