@@ -723,8 +723,8 @@ int batch_launch(lu2mi_batch *p, const double *dA, double *dX)
 
 void batch_drop_graph(lu2mi_batch *p)
 {
-    if (p->exec) hipGraphExecDestroy(p->exec);
-    if (p->graph) hipGraphDestroy(p->graph);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
     p->exec = nullptr;
     p->graph = nullptr;
     p->g_valid = 0;
@@ -822,18 +822,14 @@ extern "C" int lu2mi_batch_create(lu2mi_ctx *ctx, int nsys, const int *n, const 
 extern "C" int lu2mi_batch_destroy(lu2mi_batch *p)
 {
     if (!p) return LU2MI_OK;
-    hipSetDevice(p->ctx->device);
-    hipStreamSynchronize(p->ctx->stream);
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
     batch_drop_graph(p);
-    hipFree(p->d_sys);
-    hipFree(p->d_work);
-    hipFree(p->d_piv);
-    hipFree(p->d_resid);
-    hipFree(p->d_iters);
-    hipFree(p->d_queue);
-    hipFree(p->d_sum);
-    if (p->ev0) hipEventDestroy(p->ev0);
-    if (p->ev1) hipEventDestroy(p->ev1);
+    void *bufs[] = {p->d_sys, p->d_work, p->d_piv, p->d_resid, p->d_iters, p->d_queue, p->d_sum};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
     delete p;
     return LU2MI_OK;
 }
@@ -930,7 +926,7 @@ extern "C" int lu2mi_batch_solve(lu2mi_batch *p, const double *dA, double *dX, i
                 hipGraph_t g = nullptr;
                 const hipError_t e = hipStreamEndCapture(st, &g);
                 if (rc != LU2MI_OK) {
-                    if (g) hipGraphDestroy(g);
+                    if (g) (void)hipGraphDestroy(g);
                     return rc;
                 }
                 LU2MI_CHECK(e);

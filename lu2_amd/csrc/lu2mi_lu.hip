@@ -490,6 +490,36 @@ __global__ __launch_bounds__(256) void rowperm_kernel(PermArgs<T> a)
 }
 
 // pairs of the block's row permutation: (i+k, orig[i+k]) and (piv[i+k], orig[piv[i+k]])
+// The block-level permutation (lu2.c:524-556) without LDS staging, in two
+// launches over (pair, 1024-column chunk).  Destinations inside the block go
+// to the staging buffer W (scaled by 1/d), so pass 0 reads A and writes only
+// W.  A row below the block only ever receives content that started in the
+// block (content that enters the block from below stays there), so pass 1
+// reads block rows of A, which no pass writes, and writes rows below, which
+// pass 1 never reads.  Every read of a row below happens in pass 0.
+template <typename T>
+__global__ __launch_bounds__(256) void staged_perm_kernel(PermArgs<T> a, int pass)
+{
+    const int pr = blockIdx.y;
+    const int d = a.dst[pr];
+    const bool to_w = (d >= a.w0 && d < a.w1);
+    if (to_w != (pass == 0)) return;
+    const T *srow = a.A + (long)a.src[pr] * a.lda;
+    const bool scale = (d >= a.d0 && d < a.d1);
+    const T sc = scale ? a.invd[d] : T(1);
+    T *drow = to_w ? a.W + (long)(d - a.w0) * a.ldw - a.c0 : a.A + (long)d * a.lda;
+    const int col0 = a.c0 + blockIdx.x * 1024;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int col = col0 + q * 256 + threadIdx.x;
+        if (col < a.c1) {
+            T v = srow[col];
+            if (scale && col < a.sce) v *= sc;
+            drow[col] = v;
+        }
+    }
+}
+
 __global__ void block_pairs_kernel(const int *piv, const int *orig, int i, int nb, int *dst, int *src)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -893,17 +923,32 @@ struct Factor {
                                             hipFuncAttributeMaxDynamicSharedMemorySize, 131072));
             lds_attr = true;
         }
-        // strip width: every source row of the strip fits in 128 KB of LDS
-        int sw = (int)(131072 / (np * (int)sizeof(T)));
+        PermArgs<T> p;
+        p.A = A; p.lda = lda; p.dst = dst; p.src = src; p.np = np;
+        p.c0 = c0; p.c1 = c1; p.k0 = k0; p.k1 = k1; p.sce = sce; p.d0 = d0; p.d1 = d1;
+        p.invd = Prec<T>::invd(ctx); p.W = Wb; p.ldw = ldw; p.w0 = w0; p.w1 = w1;
+        if (Wb && w0 == d0 && w1 == d1 && k0 == k1) {
+            // block level: destinations in [w0, w1) are the block's rows
+            p.sw = 0;
+            const dim3 grid((c1 - c0 + 1023) / 1024, np);
+            hipLaunchKernelGGL(staged_perm_kernel<T>, grid, dim3(256), 0, ctx->stream, p, 0);
+            hipLaunchKernelGGL(staged_perm_kernel<T>, grid, dim3(256), 0, ctx->stream, p, 1);
+            LU2MI_CHECK(hipGetLastError());
+            return LU2MI_OK;
+        }
+        // strip width: every source row of the strip fits in LDS; strips of at
+        // least 64 bytes per row, and narrow enough for ~2 strips per CU
+        const int seg = 64 / (int)sizeof(T);
+        int sw = (c1 - c0 + 2 * ctx->num_cu - 1) / (2 * ctx->num_cu);
+        sw = (sw + seg - 1) / seg * seg;
         if (sw > 128) sw = 128;
+        const int swmax = (int)(131072 / (np * (int)sizeof(T)));
+        if (sw > swmax) sw = swmax;
         if (sw < 1) {
             lu2mi_set_error_msg("rowperm: %d row pairs do not fit in LDS", np);
             return LU2MI_ERR_UNSUPPORTED;
         }
-        PermArgs<T> p;
-        p.A = A; p.lda = lda; p.dst = dst; p.src = src; p.np = np;
-        p.c0 = c0; p.c1 = c1; p.k0 = k0; p.k1 = k1; p.sce = sce; p.d0 = d0; p.d1 = d1;
-        p.invd = Prec<T>::invd(ctx); p.W = Wb; p.ldw = ldw; p.w0 = w0; p.w1 = w1; p.sw = sw;
+        p.sw = sw;
         const int strips = (c1 - c0 + sw - 1) / sw;
         const size_t shm = (size_t)np * sw * sizeof(T);
         hipLaunchKernelGGL(rowperm_kernel<T>, dim3(strips), dim3(256), shm, ctx->stream, p);

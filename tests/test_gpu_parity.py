@@ -195,6 +195,28 @@ def test_lu_n4096_full_oracle(ctx, oracle):
     assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
 
 
+def test_lu_n16384_full_oracle(ctx, oracle):
+    """Half the config-2 size against the whole CPU oracle: the matrix is
+    generated on the device (bit-identical to the oracle's), factored with
+    nb=512, and the pivot sequence must be identical and x within 1e-10."""
+    n, nb = 16384, 512
+    lda = n + 16
+    dA = ctx.alloc(8 * n * lda)
+    dx = ctx.alloc(8 * n)
+    dp = ctx.alloc(4 * n)
+    ctx.randomsetmat(n, 1, dA, lda)
+    ctx.set_option(L.OPT_PIVMODE, L.PIV_GEPP)
+    ctx.lumcolumn_dev(n, dA, lda, nb, dp, dx)
+    x = dx.download(np.float64, n)
+    piv = dp.download(np.int32, n)
+    for d in (dA, dx, dp):
+        d.free()
+    a0 = oracle.randomsetmat(n, 1)
+    xo, po, _ = oracle.lumcolumn(a0, nb, oracle.PIV_GEPP)
+    assert np.array_equal(piv, po)
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
 def test_lu_n32768_residual(ctx):
     """Config 2 size (n=32768, nb=512): the HPL-2 scaled residual on the
     device against the regenerated matrix, and the pivot sequence shape."""
