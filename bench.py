@@ -169,12 +169,14 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=1)
-    ap.add_argument("--mixed-nb", type=int, default=1024)
+    ap.add_argument("--mixed-nb", type=int, default=2048)
     ap.add_argument("--no-lowp", action="store_true")
     ap.add_argument("--lowp-n", type=int, default=98304)
     ap.add_argument("--lowp-nb", type=int, default=2048)
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--lookahead", type=int, default=0, choices=(0, 1),
+    ap.add_argument("--leaf-w", type=int, default=16, choices=(16, 32),
+                    help="panel leaf width (columns factored per leaf launch)")
+    ap.add_argument("--lookahead", type=int, default=1, choices=(0, 1),
                     help="factor the next panel on a second stream during the trailing update")
     args = ap.parse_args()
 
@@ -195,6 +197,7 @@ def main() -> int:
     log(f"device {arch} CUs={ncu}; n={n} nb={nb} pivot={args.pivot}")
     ctx.set_option(L.OPT_PIVMODE, piv_mode)
     ctx.set_option(L.OPT_LOOKAHEAD, args.lookahead)
+    ctx.set_option(L.OPT_LEAF_W, args.leaf_w)
     nbytes = 8 * n * lda
     dA0 = ctx.alloc(nbytes)
     dA = ctx.alloc(nbytes)
@@ -282,7 +285,7 @@ def main() -> int:
         "data": "synthetic: lu2 drand48 matrix (lu2.c:70-85) generated on the device",
         "config": {"workload": f"lu2 FP64 LU with partial pivoting + solve, n={n}, nb={nb}, "
                                f"pivot={args.pivot}, 1 RHS, seed {args.seed}",
-                   "n": n, "nb": nb, "pivot": args.pivot, "lookahead": args.lookahead},
+                   "n": n, "nb": nb, "pivot": args.pivot, "lookahead": args.lookahead, "leaf_w": args.leaf_w},
         "tflops_2n3_3_plus_2n2": (flops + 2.0 * n * n) / mean / 1e12,
         "frac_of_fp64_peak": flops / mean / 1e12 / FP64_MFMA_PEAK_TFLOPS,
         "hpl2_residual": float(res[6]),

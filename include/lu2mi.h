@@ -42,10 +42,11 @@ extern "C" {
 
 /* options for lu2mi_set_option */
 #define LU2MI_OPT_PIVMODE  1   /* LU2MI_PIV_*; default COMPAT (the reference's rule) */
-#define LU2MI_OPT_LEAF_W   2   /* panel leaf width: 16 or 32 (default 32) */
+#define LU2MI_OPT_LEAF_W   2   /* panel leaf width: 16 or 32 (default 16) */
 #define LU2MI_OPT_TIMING   3   /* 1: record HIP events around trailing updates */
 #define LU2MI_OPT_LOOKAHEAD 4  /* 1: factor block i+1's panel on a second stream while
-                                  block i's trailing update runs (default 0) */
+                                  block i's trailing update runs (default 1; 0 = serial
+                                  schedule, bit-identical results) */
 
 typedef struct lu2mi_ctx lu2mi_ctx;
 

@@ -48,7 +48,7 @@ extern "C" int lu2mi_init(int device, lu2mi_ctx **out)
     if (!ctx) return LU2MI_ERR_NOMEM;
     ctx->device = device;
     ctx->pivmode = LU2MI_PIV_COMPAT;
-    ctx->leaf_w = 32;
+    ctx->leaf_w = 16;
     hipDeviceProp_t prop;
     LU2MI_CHECK(hipGetDeviceProperties(&prop, device));
     ctx->num_cu = prop.multiProcessorCount;
@@ -57,7 +57,7 @@ extern "C" int lu2mi_init(int device, lu2mi_ctx **out)
     LU2MI_CHECK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_next, hipEventDisableTiming));
     LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming));
-    ctx->lookahead = 0;
+    ctx->lookahead = 1;
     LU2MI_CHECK(hipMalloc(&ctx->d_bar, 256));
     LU2MI_CHECK(hipMemset(ctx->d_bar, 0, 256));
     LU2MI_CHECK(hipMalloc(&ctx->d_err, 256));
