@@ -870,8 +870,10 @@ struct Factor {
     int leaf(int i, int c)
     {
         const int R = n - c;
-        // rows per thread: keep the grid at <= 128 workgroups where possible
-        int rpt = (R + NTL * 128 - 1) / (NTL * 128);
+        // rows per thread: at most 64 workgroups for the 16-wide leaf (fewer
+        // slots to gather per column), 128 for the 32-wide one
+        const int gcap = (W == 16) ? 64 : 128;
+        int rpt = (R + NTL * gcap - 1) / (NTL * gcap);
         if (rpt < 1) rpt = 1;
         if (W == 32 && rpt > 2) rpt = 2;
         if (W == 16 && rpt > 4) rpt = 4;

@@ -188,3 +188,26 @@ def test_mixed_16bit_rejects_small_nb(ctx):
         ctx.solve_mixed(n, dA, lda, 32, dx, prec=L.PREC_FP16)
     for d in (dA, dx):
         d.free()
+
+
+@pytest.mark.parametrize("prec", [L.PREC_FP32, L.PREC_FP16X2])
+def test_mixed_lookahead_bit_identical(ctx, prec):
+    """The mixed solve's factorization with and without the lookahead
+    schedule performs the same arithmetic: identical iterates and x."""
+    n, nb = 2048, 256
+    lda = n + 16
+    dA = ctx.alloc(8 * n * lda)
+    ctx.randomsetmat(n, 9, dA, lda)
+    ctx.set_option(L.OPT_PIVMODE, L.PIV_GEPP)
+    xs, infos = [], []
+    for la in (0, 1):
+        dx = ctx.alloc(8 * n)
+        ctx.set_option(L.OPT_LOOKAHEAD, la)
+        infos.append(ctx.solve_mixed(n, dA, lda, nb, dx, prec=prec, maxiter=30, target=1.0))
+        xs.append(dx.download(np.float64, n))
+        dx.free()
+    ctx.set_option(L.OPT_LOOKAHEAD, 1)   # the library default
+    dA.free()
+    assert np.array_equal(xs[0], xs[1])
+    assert infos[0]["iterations"] == infos[1]["iterations"]
+    assert infos[0]["resid_hpl2_initial"] == infos[1]["resid_hpl2_initial"]

@@ -146,14 +146,15 @@ def test_lu_compat_matches_rule_fixture(ctx, oracle, n, seed):
     assert np.array_equal(piv, g["piv"])
 
 
-def test_lu_leaf16_and_wide_lda(ctx, oracle):
+@pytest.mark.parametrize("leaf_w", [16, 32])
+def test_lu_leaf_widths_and_wide_lda(ctx, oracle, leaf_w):
     n, nb = 1024, 128
     a0 = oracle.randomsetmat(n, 11, n + 40)
-    ctx.set_option(L.OPT_LEAF_W, 16)
+    ctx.set_option(L.OPT_LEAF_W, leaf_w)
     try:
         x, piv, _ = gpu_solve(ctx, a0, nb, L.PIV_GEPP)
     finally:
-        ctx.set_option(L.OPT_LEAF_W, 32)
+        ctx.set_option(L.OPT_LEAF_W, 16)   # the library default
     xo, po, _ = oracle.lumcolumn(a0, nb, oracle.PIV_GEPP)
     assert np.array_equal(piv, po)
     assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
@@ -258,6 +259,6 @@ def test_lookahead_schedule_bit_identical(ctx, n, nb, mode):
         outs.append((dA.download(np.float64, n * lda), dx.download(np.float64, n), dp.download(np.int32, n)))
         for d in (dA, dx, dp):
             d.free()
-    ctx.set_option(L.OPT_LOOKAHEAD, 0)
+    ctx.set_option(L.OPT_LOOKAHEAD, 1)   # the library default
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
