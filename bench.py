@@ -178,6 +178,9 @@ def main() -> int:
                     help="panel leaf width (columns factored per leaf launch)")
     ap.add_argument("--lookahead", type=int, default=1, choices=(0, 1),
                     help="factor the next panel on a second stream during the trailing update")
+    ap.add_argument("--subpanel", type=int, default=None,
+                    help="widest panel part factored by one persistent launch (0 = one launch per "
+                         "step; library default if omitted)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -198,6 +201,8 @@ def main() -> int:
     ctx.set_option(L.OPT_PIVMODE, piv_mode)
     ctx.set_option(L.OPT_LOOKAHEAD, args.lookahead)
     ctx.set_option(L.OPT_LEAF_W, args.leaf_w)
+    if args.subpanel is not None:
+        ctx.set_option(L.OPT_SUBPANEL, args.subpanel)
     nbytes = 8 * n * lda
     dA0 = ctx.alloc(nbytes)
     dA = ctx.alloc(nbytes)

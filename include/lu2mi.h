@@ -47,6 +47,10 @@ extern "C" {
 #define LU2MI_OPT_LOOKAHEAD 4  /* 1: factor block i+1's panel on a second stream while
                                   block i's trailing update runs (default 1; 0 = serial
                                   schedule, bit-identical results) */
+#define LU2MI_OPT_SUBPANEL 5   /* widest part of a panel (a power of two, 16..256) factored
+                                  by one persistent launch with 16-wide leaves and grid
+                                  barriers (default 0 = one launch per leaf and per GEMM,
+                                  which measured faster; results identical either way) */
 
 typedef struct lu2mi_ctx lu2mi_ctx;
 

@@ -37,6 +37,7 @@ OPT_PIVMODE = 1
 OPT_LEAF_W = 2
 OPT_TIMING = 3
 OPT_LOOKAHEAD = 4
+OPT_SUBPANEL = 5
 BATCH_EAGER = 1
 BATCH_NMAX = 512
 BATCH_RHSMAX = 8

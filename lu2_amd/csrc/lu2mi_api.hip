@@ -58,6 +58,7 @@ extern "C" int lu2mi_init(int device, lu2mi_ctx **out)
     LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_next, hipEventDisableTiming));
     LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming));
     ctx->lookahead = 1;
+    ctx->sub_w = 0;
     LU2MI_CHECK(hipMalloc(&ctx->d_bar, 256));
     LU2MI_CHECK(hipMemset(ctx->d_bar, 0, 256));
     LU2MI_CHECK(hipMalloc(&ctx->d_err, 256));
@@ -112,6 +113,13 @@ extern "C" int lu2mi_set_option(lu2mi_ctx *ctx, int opt, int value)
         return LU2MI_OK;
     case LU2MI_OPT_LOOKAHEAD:
         ctx->lookahead = value ? 1 : 0;
+        return LU2MI_OK;
+    case LU2MI_OPT_SUBPANEL:
+        if (value != 0 && (value < 16 || value > 256 || (value & (value - 1)))) {
+            lu2mi_set_error_msg("sub-panel width %d: 0 or a power of two in [16, 256]", value);
+            return LU2MI_ERR_ARG;
+        }
+        ctx->sub_w = value;
         return LU2MI_OK;
     case LU2MI_OPT_TIMING:
         ctx->timing_on = value ? 1 : 0;

@@ -68,6 +68,8 @@ struct lu2mi_ctx {
     size_t slots_bytes;
     unsigned long long *d_bar;  // back-substitution frontier (solved block count)
     unsigned long long bar_count;
+    unsigned long long gbar_count;  // sub-panel grid barrier arrivals issued so far
+    int sub_w;             // widest sub-panel factored by one persistent launch (0: off)
     int *d_err;            // device error flag
     double *d_y;           // back-substitution vector scratch
     // options

@@ -30,6 +30,7 @@
 // and scalings are coalesced along rows.
 #include "lu2mi_internal.h"
 #include "lu2mi_dev.h"
+#include "lu2mi_gemm_tile.h"
 #include <limits.h>
 #include <stdarg.h>
 #include <stdlib.h>
@@ -171,15 +172,26 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 //   column j+1 updated first, its candidate reduced in the workgroup and
 //            published (lookahead), then the other columns updated while
 //            the exchange is in flight.
+template <typename T, int W>
+struct LeafSmem {
+    T stage[NTL * (W + 1)];
+    unsigned long long xch[2][(LEAF_GMAX + 1) * (W + 3)];
+    double red_v[NTL / 64];
+    int red_r[NTL / 64];
+    int pivs[W];
+};
+
+// the leaf as workgroup bid of a.G (a stand-alone launch, or a phase of the
+// persistent sub-panel kernel)
 template <typename T, int W, int RPT>
-__global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
+__device__ __forceinline__ void leaf_body(const LeafArgs<T> &a, const int bid, LeafSmem<T, W> &sm)
 {
     constexpr int SW = W + 3;
-    __shared__ T stage[NTL * (W + 1)];
-    __shared__ unsigned long long xch[2][(LEAF_GMAX + 1) * SW];
-    __shared__ double red_v[NTL / 64];
-    __shared__ int red_r[NTL / 64];
-    __shared__ int pivs[W];
+    T *const stage = sm.stage;
+    auto &xch = sm.xch;
+    double *const red_v = sm.red_v;
+    int *const red_r = sm.red_r;
+    int *const pivs = sm.pivs;
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int G = a.G;
@@ -196,7 +208,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
     // coalesced load of the rows through LDS, one 256-row layer at a time
 #pragma clang loop unroll(full)
     for (int q = 0; q < RPT; q++) {
-        const long base = (long)c + (long)blockIdx.x * NTL + (long)q * TT;
+        const long base = (long)c + (long)bid * NTL + (long)q * TT;
         rq[q] = (int)min((long)INT_MAX, base + t);
         for (int e = t; e < NTL * W; e += NTL) {
             const int rr = e / W, kk = e % W;
@@ -247,7 +259,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
 #pragma unroll
         for (int w2 = 1; w2 < NTL / 64; w2++)
             if (better(red_v[w2], red_r[w2], wv, wr)) { wv = red_v[w2]; wr = red_r[w2]; }
-        unsigned long long *sl = slots + ((long)par * (G + 1) + blockIdx.x) * SLOT_WORDS;
+        unsigned long long *sl = slots + ((long)par * (G + 1) + bid) * SLOT_WORDS;
         auto value = [&](int q, int k) __attribute__((always_inline)) -> T {
             if (pprev && k > jc) return pending_update(row[q][k], row[q][jc > 0 ? jc - 1 : 0], (T)bitsd(pprev[3 + k]), sprev);
             return row[q][k];
@@ -270,7 +282,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
             __builtin_amdgcn_s_waitcnt(0);
             st_relaxed(sl + SLOT_WORDS - 1, seq);
         }
-        if (blockIdx.x == 0 && t == jc) {
+        if (bid == 0 && t == jc) {
             unsigned long long *cs = slots + ((long)par * (G + 1) + G) * SLOT_WORDS;
             st_relaxed(cs + 2, (unsigned long long)(unsigned)lorig[0] |
                                    ((unsigned long long)(unsigned)borig[0] << 32));
@@ -378,7 +390,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
         }
         if (t == 0) {
             pivs[jl] = p;
-            if (blockIdx.x == 0) {
+            if (bid == 0) {
                 a.piv[j] = p;
                 a.invd[j] = s;
             }
@@ -391,7 +403,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
     // ---- write back rows (coalesced through LDS), origins and row pairs
 #pragma clang loop unroll(full)
     for (int q = 0; q < RPT; q++) {
-        const long base = (long)c + (long)blockIdx.x * NTL + (long)q * TT;
+        const long base = (long)c + (long)bid * NTL + (long)q * TT;
 #pragma clang loop unroll(full)
         for (int k = 0; k < W; k++) stage[t * (W + 1) + k] = row[q][k];
         __syncthreads();
@@ -412,7 +424,7 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
                 }
         }
     }
-    if (blockIdx.x == 0) {
+    if (bid == 0) {
         if (t < W) {
             a.pair_dst[t] = c + t;
             a.pair_src[t] = lorig[0];
@@ -442,6 +454,13 @@ __global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
     }
 }
 
+template <typename T, int W, int RPT>
+__global__ __launch_bounds__(NTL) void leaf_kernel(LeafArgs<T> a)
+{
+    __shared__ LeafSmem<T, W> sm;
+    leaf_body<T, W, RPT>(a, blockIdx.x, sm);
+}
+
 // ---------------------------------------------------------------------------
 // Row permutation of a column range: all sources of a column strip are read
 // into LDS before any destination is written, so in-place permutations are
@@ -464,11 +483,9 @@ struct PermArgs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void rowperm_kernel(PermArgs<T> a)
+__device__ __forceinline__ void rowperm_strip(const PermArgs<T> &a, const int strip, T *buf)
 {
-    extern __shared__ unsigned char smem_raw[];
-    T *buf = (T *)smem_raw;
-    const int cs = a.c0 + blockIdx.x * a.sw;
+    const int cs = a.c0 + strip * a.sw;
     const int total = a.np * a.sw;
     for (int e = threadIdx.x; e < total; e += 256) {
         const int pr = e / a.sw, cc = e % a.sw;
@@ -487,6 +504,13 @@ __global__ __launch_bounds__(256) void rowperm_kernel(PermArgs<T> a)
             else a.A[(long)d * a.lda + col] = v;
         }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void rowperm_kernel(PermArgs<T> a)
+{
+    extern __shared__ unsigned char smem_raw[];
+    rowperm_strip<T>(a, blockIdx.x, (T *)smem_raw);
 }
 
 // pairs of the block's row permutation: (i+k, orig[i+k]) and (piv[i+k], orig[piv[i+k]])
@@ -543,6 +567,163 @@ __global__ void scale_copy_kernel(const T *A, long lda, int r0, int c0, int h, i
         W[e] = A[(long)(r0 + r) * lda + c0 + col] * invd[r0 + r];
     }
 }
+
+// ---------------------------------------------------------------------------
+// Persistent sub-panel: one launch factors w columns [c, c+w) of the block's
+// panel (w a power of two <= SUB_WMAX, 16-column leaves): every leaf, its row
+// swaps over the panel's other columns, and every recursion node inside the
+// sub-panel (1/d scaling and staging, explicit-inverse TRSM, update of the
+// rows below, off-diagonal block of the inverse), in the order of the
+// recursion of lu2.c:1187-1235, separated by grid barriers instead of kernel
+// boundaries.  The steps run the same device code as the separate launches
+// (leaf body, row-swap strip, GEMM tile), so the factors are bit-identical;
+// the difference is one dispatch instead of about seven per leaf, which is
+// what a panel factored beside the trailing update waits on.
+constexpr int SUB_W = 16;       // leaf width inside the sub-panel kernel
+constexpr int SUB_WMAX = 256;   // widest sub-panel
+constexpr int SUB_SWMAX = 128;  // widest row-swap strip (LDS: 2*SUB_W pairs x strip)
+constexpr size_t SLOTS_BYTES = sizeof(unsigned long long) * 2 * (size_t)(1024 + 1) * SLOT_WORDS;
+
+template <typename T>
+struct SubArgs {
+    LeafArgs<T> leaf;            // first leaf: c = sub-panel start, seq_base of its first column
+    int nb, w, sw;               // block size, sub-panel width, row-swap strip width
+    T *wbuf, *tmp;               // h x h staging of the scaled rows; h x h product scratch
+    unsigned long long *gbar;    // grid barrier arrivals (monotonic within a factorization)
+    unsigned long long gbar_base;
+};
+
+// Barrier over the G co-resident workgroups: arrivals counted on one word,
+// agent-scope fences on both sides so that every step's global writes (on
+// any XCD) are visible to the next step.  Bounded: a timeout raises *err and
+// lets every wave run to the end of the kernel.
+__device__ __forceinline__ void grid_sync(unsigned long long *ctr, unsigned long long target, int *err)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(ctr, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long spins = 0;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (((++spins) & 1023) == 0) {
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (spins > (1UL << 25)) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// C = alpha A B + beta C over the G workgroups in 64x64 tiles, each with the
+// K range of the stand-alone GEMM's 128x128 tile that contains it
+template <typename T, int EPI>
+__device__ __forceinline__ void gemm_steps(int M, int N, int K, double alpha, const T *A, long lda, const T *B,
+                                           long ldb, double beta, T *C, long ldc, int a_lower, int bid, int G,
+                                           lu2mi_gemm::Tile64Smem<T> &ts)
+{
+    using namespace lu2mi_gemm;
+    GemmDesc g;
+    g.M = M; g.N = N; g.K = K; g.alpha = alpha; g.beta = beta;
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc; g.a_lower = a_lower;
+    const int tiles_m = (M + BT64 - 1) / BT64, tiles_n = (N + BT64 - 1) / BT64;
+    const int total = (M > 0 && N > 0) ? tiles_m * tiles_n : 0;
+    for (int tile = bid; tile < total; tile += G) {
+        const int m0 = (tile / tiles_n) * BT64, n0 = (tile % tiles_n) * BT64;
+        const int Kt = a_lower ? min(K, (m0 & ~(BM - 1)) + BM) : K;
+        const bool interior = (m0 + BT64 <= M) && (n0 + BT64 <= N) && (Kt % Mfma<T>::BK == 0);
+        if (interior) gemm_tile64<T, false, EPI>(g, A, B, C, m0, n0, Kt, ts);
+        else gemm_tile64<T, true, EPI>(g, A, B, C, m0, n0, Kt, ts);
+        __syncthreads();   // the next tile restages the same LDS
+    }
+}
+
+template <typename T, int RPT>
+__global__ __launch_bounds__(NTL, 2) void subpanel_kernel(SubArgs<T> s)
+{
+    using namespace lu2mi_gemm;
+    constexpr int W = SUB_W;
+    __shared__ union {
+        LeafSmem<T, W> leaf;
+        Tile64Smem<T> tile;
+        T perm[2 * W * SUB_SWMAX];
+    } sm;
+    const int bid = blockIdx.x, G = s.leaf.G;
+    const int i = s.leaf.i, n = s.leaf.n, c0 = s.leaf.c, nb = s.nb, L = s.w / W;
+    const long lda = s.leaf.lda;
+    const int ldl = s.leaf.ldl;
+    T *const A = s.leaf.A;
+    T *const linv = s.leaf.linv;
+    unsigned long long target = s.gbar_base;
+    auto sync = [&]() __attribute__((always_inline)) {
+        target += (unsigned long long)G;
+        grid_sync(s.gbar, target, s.leaf.err);
+    };
+    // node (cn, 2h) after its left half: rows [cn, cn+h) of the right half
+    // scaled by 1/d and staged, X = inv(D^-1 L11) * staged rows, then the
+    // rows below (host path: Factor::panel)
+    auto node_mid = [&](const int cn, const int h) __attribute__((always_inline)) {
+        const long off = cn - i;
+        for (long e = (long)bid * NTL + threadIdx.x; e < (long)h * h; e += (long)G * NTL) {
+            const int r = (int)(e / h), col = (int)(e % h);
+            s.wbuf[e] = A[(long)(cn + r) * lda + cn + h + col] * s.leaf.invd[cn + r];
+        }
+        sync();
+        gemm_steps<T, EPI_SET>(h, h, h, 1.0, linv + off * ldl + off, ldl, s.wbuf, h, 0.0,
+                               A + (long)cn * lda + cn + h, lda, 1, bid, G, sm.tile);
+        sync();
+        gemm_steps<T, EPI_SUB>(n - cn - h, h, h, -1.0, A + (long)(cn + h) * lda + cn, lda,
+                               A + (long)cn * lda + cn + h, lda, 1.0, A + (long)(cn + h) * lda + cn + h, lda, 0,
+                               bid, G, sm.tile);
+        sync();
+    };
+    // node (cn, 2h) after its right half: X21 = -inv22 * L21 * inv11
+    auto node_end = [&](const int cn, const int h) __attribute__((always_inline)) {
+        const long off = cn - i;
+        gemm_steps<T, EPI_SET>(h, h, h, 1.0, A + (long)(cn + h) * lda + cn, lda, linv + off * ldl + off, ldl, 0.0,
+                               s.tmp, h, 0, bid, G, sm.tile);
+        sync();
+        gemm_steps<T, EPI_GENERAL>(h, h, h, -1.0, linv + (off + h) * ldl + off + h, ldl, s.tmp, h, 0.0,
+                                   linv + (off + h) * ldl + off, ldl, 1, bid, G, sm.tile);
+        sync();
+    };
+    // post-order walk: before leaf k the nodes whose right half ended at
+    // leaf k-1 (deepest first) get their inverse block, then the node whose
+    // left half ended there gets its right half updated
+    for (int k = 0; k <= L; k++) {
+        if (k > 0) {
+            const int tz = __builtin_ctz(k);
+            for (int lv = 1; lv <= tz; lv++) node_end(c0 + (k - (1 << lv)) * W, W << (lv - 1));
+            if (k == L) break;
+            node_mid(c0 + (k - (1 << tz)) * W, W << tz);
+        }
+        LeafArgs<T> la = s.leaf;
+        la.c = c0 + k * W;
+        la.seq_base = s.leaf.seq_base + (unsigned long long)k * W;
+        leaf_body<T, W, RPT>(la, bid, sm.leaf);
+        sync();
+        // the leaf's swaps on the panel's other columns; its diagonal rows'
+        // L part (columns left of the leaf) scaled by 1/d
+        PermArgs<T> p;
+        p.A = A; p.lda = lda; p.dst = la.pair_dst; p.src = la.pair_src; p.np = 2 * W;
+        p.c0 = i; p.c1 = i + nb; p.k0 = la.c; p.k1 = la.c + W; p.sce = la.c; p.d0 = la.c; p.d1 = la.c + W;
+        p.invd = la.invd; p.W = nullptr; p.ldw = 0; p.w0 = 0; p.w1 = 0; p.sw = s.sw;
+        const int strips = (nb + s.sw - 1) / s.sw;
+        for (int st = bid; st < strips; st += G) {
+            rowperm_strip<T>(p, st, sm.perm);
+            __syncthreads();
+        }
+        sync();
+    }
+}
+
+// grid barriers one sub-panel launch of L leaves passes: two per leaf, three
+// per node before its right half, two per node after it
+__host__ __device__ constexpr int sub_barriers(int L) { return 2 * L + 5 * (L - 1); }
 
 // column-major panel (m columns of ld rows, rows [r0, r0+R)) <-> row-major (R x ldp)
 __global__ void cm_to_rm_kernel(const double *cm, long ld, int r0, int R, int m, double *rm, long ldp, int back)
@@ -797,13 +978,18 @@ int lu2mi_ctx_reserve(lu2mi_ctx *ctx, int n, int nb, int esz)
     LU2MI_CHECK(hipMalloc(&ctx->d_orig, sizeof(int) * (size_t)ncap));
     LU2MI_CHECK(hipMalloc(&ctx->d_invd, sizeof(double) * (size_t)ncap));
     LU2MI_CHECK(hipMalloc(&ctx->d_invd_f, sizeof(float) * (size_t)ncap));
-    LU2MI_CHECK(hipMalloc(&ctx->d_pairs, sizeof(int) * 4 * (size_t)(nbcap > 64 ? nbcap : 64)));
-    LU2MI_CHECK(hipMalloc(&ctx->d_linv, sizeof(double) * (size_t)nbcap * nbcap));
-    LU2MI_CHECK(hipMalloc(&ctx->d_tmp, sizeof(double) * (size_t)nbcap * nbcap));
+    // pairs and Linv: two of each (blocks i and i+1 alternate, so the next
+    // panel can be factored while block i's rows are still being moved);
+    // d_tmp: the panel's h x h product scratch, then its h x h staging rows
+    LU2MI_CHECK(hipMalloc(&ctx->d_pairs, sizeof(int) * 2 * 4 * (size_t)(nbcap > 64 ? nbcap : 64)));
+    LU2MI_CHECK(hipMalloc(&ctx->d_linv, sizeof(double) * 2 * (size_t)nbcap * nbcap));
+    LU2MI_CHECK(hipMalloc(&ctx->d_tmp, sizeof(double) * 2 * (size_t)nbcap * nbcap));
     LU2MI_CHECK(hipMalloc(&ctx->d_y, sizeof(double) * 4 * (size_t)ncap + 64));
     size_t wneed = sizeof(double) * (size_t)nbcap * (size_t)(ncap + 1);
     LU2MI_TRY(lu2mi_dev_buf(&ctx->d_w, &ctx->w_bytes, wneed));
-    size_t sneed = sizeof(unsigned long long) * 2 * (size_t)(1024 + 1) * SLOT_WORDS;
+    // leaf exchange slots, then the sub-panel grid barrier word (both cleared
+    // at the start of every factorization)
+    size_t sneed = SLOTS_BYTES + 256;
     LU2MI_TRY(lu2mi_dev_buf(&ctx->d_slots, &ctx->slots_bytes, sneed));
     ctx->n_cap = ncap;
     ctx->nb_cap = nbcap;
@@ -861,7 +1047,10 @@ struct Factor {
     long lda;
     int n, nb, compat, W;
     T *linv, *tmp, *wbuf;
+    T *pwbuf;              // the panel's staging rows (wbuf holds the block's)
     int *pairs;
+    T *linv_base = nullptr;   // two nb x nb inverses, alternating by block
+    int *pairs_base = nullptr;
     // when set, every block keeps its inverse diagonal block and its row
     // pairs here (for later solves with the stored factors)
     T *linv_all = nullptr;
@@ -958,8 +1147,71 @@ struct Factor {
         return LU2MI_OK;
     }
 
+    // rows per thread and workgroups of a 16-wide leaf over R rows: at most
+    // 64 workgroups (fewer slots to gather per column), up to 4 rows each
+    static void leaf16_shape(int R, int &rpt, int &G)
+    {
+        rpt = (R + NTL * 64 - 1) / (NTL * 64);
+        if (rpt < 1) rpt = 1;
+        if (rpt > 4) rpt = 4;
+        if (rpt == 3) rpt = 4;
+        G = (R + NTL * rpt - 1) / (NTL * rpt);
+    }
+
+    // the sub-panel kernel applies to 16-wide leaves on 16-byte aligned rows
+    bool sub_ok(int w) const
+    {
+        constexpr long EPV = 16 / sizeof(T);
+        auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+        return ctx->sub_w > 0 && W == SUB_W && w <= ctx->sub_w && w <= SUB_WMAX && w >= W && al(A) &&
+               lda % EPV == 0 && nb % EPV == 0 && al(linv) && al(pwbuf) && al(tmp);
+    }
+
+    int subpanel(int i, int c, int w)
+    {
+        int rpt, G;
+        leaf16_shape(n - c, rpt, G);
+        if (G > LEAF_GMAX) {
+            lu2mi_set_error_msg("sub-panel: %d rows need %d workgroups (> %d)", n - c, G, LEAF_GMAX);
+            return LU2MI_ERR_UNSUPPORTED;
+        }
+        SubArgs<T> sa;
+        LeafArgs<T> &la = sa.leaf;
+        la.A = A; la.lda = lda; la.n = n; la.c = c; la.i = i; la.compat = compat; la.G = G;
+        la.piv = ctx->d_piv; la.orig = ctx->d_orig; la.invd = Prec<T>::invd(ctx);
+        la.pair_dst = pairs; la.pair_src = pairs + 2 * SUB_W;
+        la.linv = linv; la.ldl = nb;
+        la.slots = (unsigned long long *)ctx->d_slots;
+        la.seq_base = ctx->bar_count; la.err = ctx->d_err;
+        sa.nb = nb; sa.w = w;
+        // strips of at least 64 bytes per row, about one per workgroup
+        const int seg = 64 / (int)sizeof(T);
+        int sw = (nb + G - 1) / G;
+        sw = (sw + seg - 1) / seg * seg;
+        sa.sw = sw > SUB_SWMAX ? SUB_SWMAX : sw;
+        sa.wbuf = pwbuf; sa.tmp = tmp;
+        sa.gbar = (unsigned long long *)((char *)ctx->d_slots + SLOTS_BYTES);
+        sa.gbar_base = ctx->gbar_count;
+        const void *fn = (rpt == 1) ? (const void *)subpanel_kernel<T, 1>
+                         : (rpt == 2) ? (const void *)subpanel_kernel<T, 2> : (const void *)subpanel_kernel<T, 4>;
+        // all G workgroups must be co-resident (grid barriers), as for the leaf
+        int per_cu = 0;
+        LU2MI_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTL, 0));
+        if ((long)per_cu * ctx->num_cu < G) {
+            lu2mi_set_error_msg("sub-panel: %d workgroups cannot be co-resident (%d per CU x %d CUs)", G, per_cu,
+                                ctx->num_cu);
+            return LU2MI_ERR_UNSUPPORTED;
+        }
+        void *args[] = {&sa};
+        LU2MI_CHECK(hipLaunchKernel(fn, dim3(G), dim3(NTL), args, 0, ctx->stream));
+        ctx->bar_count += (unsigned long long)w;   // W sequence numbers per leaf
+        ctx->gbar_count += (unsigned long long)G * sub_barriers(w / SUB_W);
+        return LU2MI_OK;
+    }
+
     int panel(int i, int c, int w)
     {
+        if (sub_ok(w)) return subpanel(i, c, w);
         if (w <= W) return leaf(i, c);
         const int h = w / 2;
         LU2MI_TRY(panel(i, c, h));
@@ -968,10 +1220,10 @@ struct Factor {
         {
             const long e = (long)h * h;
             hipLaunchKernelGGL(scale_copy_kernel<T>, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, ctx->stream,
-                               A, lda, c, c + h, h, h, Prec<T>::invd(ctx), wbuf);
+                               A, lda, c, c + h, h, h, Prec<T>::invd(ctx), pwbuf);
             LU2MI_CHECK(hipGetLastError());
             const long off = (long)(c - i);
-            LU2MI_TRY(gemm_call<T>(ctx, h, h, h, 1.0, linv + off * nb + off, nb, wbuf, h, 0.0,
+            LU2MI_TRY(gemm_call<T>(ctx, h, h, h, 1.0, linv + off * nb + off, nb, pwbuf, h, 0.0,
                                    A + (long)c * lda + c + h, lda, 1));
             LU2MI_TRY(gemm_call<T>(ctx, n - c - h, h, h, -1.0, A + (long)(c + h) * lda + c, lda,
                                    A + (long)c * lda + c + h, lda, 1.0, A + (long)(c + h) * lda + c + h, lda, 0));
@@ -994,6 +1246,9 @@ struct Factor {
         if (linv_all) {
             linv = linv_all + (long)(i / nb) * nb * nb;
             pairs = pairs_all + (long)(i / nb) * 4 * nb;
+        } else if (linv_base) {
+            linv = linv_base + (long)((i / nb) & 1) * nb * nb;
+            pairs = pairs_base + (long)((i / nb) & 1) * 4 * nb;
         }
     }
 
@@ -1027,12 +1282,13 @@ struct Factor {
         return LU2MI_OK;
     }
 
-    // Block loop of lu2.c:1122-1144.  With lookahead, block i's trailing
-    // update is split: the next panel's nb columns first, then the rest; the
-    // next panel is factored on the second stream as soon as its columns are
-    // up to date, while the rest of the update runs.  The two touch disjoint
-    // columns, and block i's pairs, staging rows and Linv are consumed before
-    // the split, so the panel may reuse them.
+    // Block loop of lu2.c:1122-1144.  With lookahead, block i's row
+    // permutation, U12 product and trailing update are each split: the next
+    // panel's nb columns first, then the rest; the next panel is factored on
+    // the second stream as soon as its columns are up to date, while the rest
+    // of block i runs.  The two touch disjoint columns; the next panel has its
+    // own Linv and pair buffers (they alternate by block) and its own staging
+    // rows, so block i's stay intact until its rest is done.
     int run(int *dpiv_out, T *dx)
     {
         const bool la = ctx->lookahead != 0;
@@ -1045,19 +1301,22 @@ struct Factor {
                 panel_pending = false;
             }
             set_block(i);
+            T *const linv_i = linv;
+            int *const pairs_i = pairs;
             const int ncols = n + 1 - (i + nb);
+            const int M = n - i - nb;
+            const int na = (la && M > 0) ? nb : ncols;   // columns before the split
             // block-level row permutation over [i+nb, n], 1/d scaling of the
             // block rows, block rows staged in w (lu2.c:502-509 awork copy)
             hipLaunchKernelGGL(block_pairs_kernel, dim3((nb + 255) / 256), dim3(256), 0, s1, ctx->d_piv,
-                               ctx->d_orig, i, nb, pairs, pairs + 2 * nb);
+                               ctx->d_orig, i, nb, pairs_i, pairs_i + 2 * nb);
             LU2MI_CHECK(hipGetLastError());
             const long ldw = (ncols + 3) & ~3L;   // 16-byte rows for FP64 and FP32
-            LU2MI_TRY(rowperm(pairs, pairs + 2 * nb, 2 * nb, i + nb, n + 1, 0, 0, n + 1, i, i + nb, wbuf, ldw, i,
-                              i + nb));
+            LU2MI_TRY(rowperm(pairs_i, pairs_i + 2 * nb, 2 * nb, i + nb, i + nb + na, 0, 0, n + 1, i, i + nb, wbuf,
+                              ldw, i, i + nb));
             // U12 = inv(D^-1 L11) * staged rows   (lu2.c:511-512)
-            LU2MI_TRY(gemm_call<T>(ctx, nb, ncols, nb, 1.0, linv, nb, wbuf, ldw, 0.0, A + (long)i * lda + i + nb,
+            LU2MI_TRY(gemm_call<T>(ctx, nb, na, nb, 1.0, linv_i, nb, wbuf, ldw, 0.0, A + (long)i * lda + i + nb,
                                    lda, 1));
-            const int M = n - i - nb;
             if (M <= 0) continue;
             if (la) {
                 LU2MI_TRY(trailing(i, i + nb, nb));                    // the next panel's columns
@@ -1069,7 +1328,13 @@ struct Factor {
                 LU2MI_TRY(prc);
                 LU2MI_CHECK(hipEventRecord(ctx->ev_panel, ctx->stream2));
                 panel_pending = true;
-                LU2MI_TRY(trailing(i, i + 2 * nb, ncols - nb));       // the rest, b included
+                // the rest of block i, b included: staging rows from column nb
+                // of w on, so that w keeps the layout of the unsplit product
+                LU2MI_TRY(rowperm(pairs_i, pairs_i + 2 * nb, 2 * nb, i + 2 * nb, n + 1, 0, 0, n + 1, i, i + nb,
+                                  wbuf + nb, ldw, i, i + nb));
+                LU2MI_TRY(gemm_call<T>(ctx, nb, ncols - nb, nb, 1.0, linv_i, nb, wbuf + nb, ldw, 0.0,
+                                       A + (long)i * lda + i + 2 * nb, lda, 1));
+                LU2MI_TRY(trailing(i, i + 2 * nb, ncols - nb));
             } else {
                 LU2MI_TRY(trailing(i, i + nb, ncols));
                 LU2MI_TRY(start_panel(i + nb));
@@ -1105,6 +1370,8 @@ static int factor_impl(lu2mi_ctx *ctx, int n, T *dA, long lda, int nb, int pivmo
     // a W=32 leaf holds at most 2 rows per thread in registers over LEAF_GMAX workgroups
     if (f.W == 32 && n > 2 * NTL * LEAF_GMAX) f.W = 16;
     f.linv = (T *)ctx->d_linv; f.tmp = (T *)ctx->d_tmp; f.wbuf = (T *)ctx->d_w; f.pairs = ctx->d_pairs;
+    f.pwbuf = (T *)((double *)ctx->d_tmp + (size_t)ctx->nb_cap * ctx->nb_cap);
+    f.linv_base = (T *)ctx->d_linv; f.pairs_base = ctx->d_pairs;
     f.linv_all = keep_linv;
     f.pairs_all = keep_pairs;
     ctx->tgemm.used = 0;
@@ -1112,6 +1379,7 @@ static int factor_impl(lu2mi_ctx *ctx, int n, T *dA, long lda, int nb, int pivmo
     // slot sequence numbers restart with every factorization
     LU2MI_CHECK(hipMemsetAsync(ctx->d_slots, 0, ctx->slots_bytes, ctx->stream));
     ctx->bar_count = 0;
+    ctx->gbar_count = 0;
     auto t0 = std::chrono::steady_clock::now();
     int rc = f.run(dpiv_out, x);
     if (rc != LU2MI_OK) return rc;
@@ -1165,12 +1433,14 @@ int lu2mi_cm_panel_f64(lu2mi_ctx *ctx, double *dcm, long ld, int n, int m, int i
     Factor<double> f;
     f.ctx = ctx; f.A = P; f.lda = ldp; f.n = R; f.nb = m; f.compat = (pivmode == LU2MI_PIV_COMPAT); f.W = m;
     f.linv = (double *)ctx->d_linv; f.tmp = (double *)ctx->d_tmp; f.wbuf = (double *)ctx->d_w;
+    f.pwbuf = (double *)ctx->d_tmp + (size_t)ctx->nb_cap * ctx->nb_cap;
     f.pairs = ctx->d_pairs;
     int rc = LU2MI_OK;
     if (hipMemsetAsync(ctx->d_err, 0, sizeof(int), ctx->stream) != hipSuccess ||
         hipMemsetAsync(ctx->d_slots, 0, ctx->slots_bytes, ctx->stream) != hipSuccess)
         rc = LU2MI_ERR_HIP;
     ctx->bar_count = 0;
+    ctx->gbar_count = 0;
     if (rc == LU2MI_OK) rc = f.panel(0, 0, m);
     if (rc == LU2MI_OK) {
         hipLaunchKernelGGL(cm_to_rm_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, dcm, ld,

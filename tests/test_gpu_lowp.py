@@ -211,3 +211,26 @@ def test_mixed_lookahead_bit_identical(ctx, prec):
     assert np.array_equal(xs[0], xs[1])
     assert infos[0]["iterations"] == infos[1]["iterations"]
     assert infos[0]["resid_hpl2_initial"] == infos[1]["resid_hpl2_initial"]
+
+
+@pytest.mark.parametrize("prec", [L.PREC_FP32, L.PREC_FP16X2])
+def test_mixed_subpanel_bit_identical(ctx, prec):
+    """The FP32 factorization inside the mixed solve with the persistent
+    sub-panel launch and with one launch per step: identical iterates and x."""
+    n, nb = 2048, 512
+    lda = n + 16
+    dA = ctx.alloc(8 * n * lda)
+    ctx.randomsetmat(n, 11, dA, lda)
+    ctx.set_option(L.OPT_PIVMODE, L.PIV_GEPP)
+    xs, infos = [], []
+    for sw in (0, 256):
+        dx = ctx.alloc(8 * n)
+        ctx.set_option(L.OPT_SUBPANEL, sw)
+        infos.append(ctx.solve_mixed(n, dA, lda, nb, dx, prec=prec, maxiter=30, target=1.0))
+        xs.append(dx.download(np.float64, n))
+        dx.free()
+    ctx.set_option(L.OPT_SUBPANEL, 0)   # the library default
+    dA.free()
+    assert np.array_equal(xs[0], xs[1])
+    assert infos[0]["iterations"] == infos[1]["iterations"]
+    assert infos[0]["resid_hpl2_initial"] == infos[1]["resid_hpl2_initial"]

@@ -262,3 +262,43 @@ def test_lookahead_schedule_bit_identical(ctx, n, nb, mode):
     ctx.set_option(L.OPT_LOOKAHEAD, 1)   # the library default
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,nb,mode,sub", [
+    (1024, 256, L.PIV_COMPAT, 256),
+    (2048, 512, L.PIV_GEPP, 256),
+    (2048, 128, L.PIV_GEPP, 64),
+    (4096, 1024, L.PIV_COMPAT, 128),
+    (1536, 512, L.PIV_GEPP, 32),
+    (20480, 2048, L.PIV_GEPP, 256),
+])
+def test_subpanel_kernel_bit_identical(ctx, n, nb, mode, sub):
+    """The persistent sub-panel launch (leaves, their swaps and the recursion
+    nodes of up to 256 columns between grid barriers) performs the same
+    arithmetic as one launch per step: factors, pivots and x bit-identical,
+    with and without the lookahead schedule."""
+    lda = n + 16
+    outs = []
+    for sw, la in ((0, 0), (sub, 0), (sub, 1)):
+        dA = ctx.alloc(8 * n * lda)
+        dx = ctx.alloc(8 * n)
+        dp = ctx.alloc(4 * n)
+        ctx.randomsetmat(n, 5, dA, lda)
+        ctx.set_option(L.OPT_PIVMODE, mode)
+        ctx.set_option(L.OPT_SUBPANEL, sw)
+        ctx.set_option(L.OPT_LOOKAHEAD, la)
+        ctx.lumcolumn_dev(n, dA, lda, nb, dp, dx)
+        outs.append((dA.download(np.float64, n * lda), dx.download(np.float64, n), dp.download(np.int32, n)))
+        for d in (dA, dx, dp):
+            d.free()
+    ctx.set_option(L.OPT_SUBPANEL, 0)   # the library defaults
+    ctx.set_option(L.OPT_LOOKAHEAD, 1)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
+
+
+def test_subpanel_option_rejects(ctx):
+    for bad in (8, 48, 512, -1):
+        with pytest.raises(L.Lu2miError):
+            ctx.set_option(L.OPT_SUBPANEL, bad)
