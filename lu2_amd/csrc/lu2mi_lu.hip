@@ -25,6 +25,13 @@
 //    A22 -= L21 * U12                     (lu2.c:580, 97.7% of the flops)
 //  back substitution on the unit-upper U (lu2.c:110-116), blocked.
 //
+//  Lookahead schedule (default): the block-level steps run first on the next
+//  panel's nb columns only, panel(i+nb) then starts on a second stream, and
+//  the block's remaining columns follow on the first; same arithmetic per
+//  element as the serial order.  Optionally, up to 256 columns of a panel
+//  run as one persistent launch (subpanel_kernel) with grid barriers between
+//  the steps above.
+//
 // Layout in HBM: lu2's row-major augmented matrix, lda >= n+1 (RDIM = n+16
 // in lu2.c:28), b in column n.  Row-major rows are contiguous, so row swaps
 // and scalings are coalesced along rows.
