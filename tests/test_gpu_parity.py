@@ -102,8 +102,11 @@ def test_gemm_strided_submatrix(ctx):
 
 
 # ----------------------------------------------------------------------- LU
+# (256, 256): one block; (512, 256): two blocks, so the split schedule's
+# remaining columns are b alone; (64, 32): the smallest block
 CASES = [(256, 64, 1), (512, 64, 12345), (1024, 128, 1), (1024, 256, 12345),
-         (2048, 256, 1), (2048, 512, 12345), (2048, 128, 7), (512, 32, 3)]
+         (2048, 256, 1), (2048, 512, 12345), (2048, 128, 7), (512, 32, 3),
+         (256, 256, 5), (512, 256, 9), (64, 32, 2)]
 
 
 @pytest.mark.parametrize("n,nb,seed", CASES)
