@@ -290,7 +290,8 @@ def main() -> int:
         "data": "synthetic: lu2 drand48 matrix (lu2.c:70-85) generated on the device",
         "config": {"workload": f"lu2 FP64 LU with partial pivoting + solve, n={n}, nb={nb}, "
                                f"pivot={args.pivot}, 1 RHS, seed {args.seed}",
-                   "n": n, "nb": nb, "pivot": args.pivot, "lookahead": args.lookahead, "leaf_w": args.leaf_w},
+                   "n": n, "nb": nb, "pivot": args.pivot, "lookahead": args.lookahead, "leaf_w": args.leaf_w,
+                   "subpanel": args.subpanel if args.subpanel is not None else 0},
         "tflops_2n3_3_plus_2n2": (flops + 2.0 * n * n) / mean / 1e12,
         "frac_of_fp64_peak": flops / mean / 1e12 / FP64_MFMA_PEAK_TFLOPS,
         "hpl2_residual": float(res[6]),
