@@ -15,6 +15,25 @@
 using namespace lu2mi_tiles;
 using namespace lu2mi_gemm;
 
+// Narrow products (M or N <= 64: the recursion nodes near the leaves, their
+// TRSM and inverse blocks) in 64x64 tiles, which waste less of the matrix
+// cores on padding; every element keeps the K range of the 128x128 tile
+// that contains it, so the results are the 128x128 kernel's bits.
+template <typename T, int EPI>
+__global__ __launch_bounds__(NT) void gemm64_kernel(GemmDesc g, int tiles_m, int tiles_n)
+{
+    __shared__ Tile64Smem<T> sm;
+    const int m0 = (blockIdx.x / tiles_n) * BT64, n0 = (blockIdx.x % tiles_n) * BT64;
+    const long bz = blockIdx.y;
+    const T *A = (const T *)g.A + bz * g.sA;
+    const T *B = (const T *)g.B + bz * g.sB;
+    T *C = (T *)g.C + bz * g.sC;
+    const int K = g.a_lower ? min(g.K, (m0 & ~(BM - 1)) + BM) : g.K;
+    const bool interior = (m0 + BT64 <= g.M) && (n0 + BT64 <= g.N) && (K % Mfma<T>::BK == 0);
+    if (interior) gemm_tile64<T, false, EPI>(g, A, B, C, m0, n0, K, sm);
+    else gemm_tile64<T, true, EPI>(g, A, B, C, m0, n0, K, sm);
+}
+
 template <typename T, bool VEC, int EPI, int TAG>
 __global__ __launch_bounds__(NT) void gemm_kernel(GemmDesc g, int tiles_m, int tiles_n)
 {
@@ -61,6 +80,18 @@ static int launch_gemm(const GemmDesc &g, hipStream_t s)
                      (g.sA % EPV == 0) && (g.sB % EPV == 0);
     const int epi = (g.alpha == -1.0 && g.beta == 1.0) ? EPI_SUB : (g.alpha == 1.0 && g.beta == 0.0) ? EPI_SET
                                                                                                    : EPI_GENERAL;
+    if (vec && g.tag == 0 && (g.M <= BT64 || g.N <= BT64)) {
+        const int tm64 = (g.M + BT64 - 1) / BT64, tn64 = (g.N + BT64 - 1) / BT64;
+        dim3 grid64((unsigned)(tm64 * tn64), (unsigned)g.batch), block64(NT);
+        if (epi == EPI_SUB)
+            hipLaunchKernelGGL((gemm64_kernel<T, EPI_SUB>), grid64, block64, 0, s, g, tm64, tn64);
+        else if (epi == EPI_SET)
+            hipLaunchKernelGGL((gemm64_kernel<T, EPI_SET>), grid64, block64, 0, s, g, tm64, tn64);
+        else
+            hipLaunchKernelGGL((gemm64_kernel<T, EPI_GENERAL>), grid64, block64, 0, s, g, tm64, tn64);
+        LU2MI_CHECK(hipGetLastError());
+        return LU2MI_OK;
+    }
     dim3 grid((unsigned)total, (unsigned)g.batch), block(NT);
     if (vec && epi == EPI_SUB && g.tag == 1)
         hipLaunchKernelGGL((gemm_kernel<T, true, EPI_SUB, 1>), grid, block, 0, s, g, tiles_m, tiles_n);

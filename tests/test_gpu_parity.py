@@ -101,6 +101,30 @@ def test_gemm_strided_submatrix(ctx):
     assert np.array_equal(got[:300], big[:300])
 
 
+@pytest.mark.parametrize("m,n,k,narrow", [(1000, 64, 300, "n"), (1000, 16, 48, "n"), (64, 900, 200, "m"),
+                                           (40, 700, 130, "m")])
+def test_gemm_narrow_tiles_bit_identical(ctx, m, n, k, narrow):
+    """Products with M or N <= 64 run in 64x64 tiles; each element gets the
+    same MFMA sequence as in the 128x128 tile of a wider product."""
+    rng = np.random.default_rng(m + n + k)
+    wide_m, wide_n = (m, 200) if narrow == "n" else (200, n)
+    a = rng.standard_normal((wide_m, k))
+    b = rng.standard_normal((k, wide_n))
+    c = rng.standard_normal((wide_m, wide_n))
+    out = {}
+    for tag, (mm, nn) in (("wide", (wide_m, wide_n)), ("narrow", (m, n))):
+        dA, dB, dC = ctx.alloc(8 * mm * k), ctx.alloc(8 * k * nn), ctx.alloc(8 * mm * nn)
+        dA.upload(np.ascontiguousarray(a[:mm]))
+        dB.upload(np.ascontiguousarray(b[:, :nn]))
+        dC.upload(np.ascontiguousarray(c[:mm, :nn]))
+        ctx.dgemm_dev(mm, nn, k, -1.0, dA, k, dB, nn, 1.0, dC, nn)
+        out[tag] = dC.download(np.float64, mm * nn).reshape(mm, nn)
+        for d in (dA, dB, dC):
+            d.free()
+    assert np.array_equal(out["wide"][:m, :n], out["narrow"])
+    assert np.abs(out["narrow"] - (c[:m, :n] - a[:m] @ b[:, :n])).max() < 1e-12 * k
+
+
 # ----------------------------------------------------------------------- LU
 # (256, 256): one block; (512, 256): two blocks, so the split schedule's
 # remaining columns are b alone; (64, 32): the smallest block
