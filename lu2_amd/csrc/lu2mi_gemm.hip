@@ -16,9 +16,11 @@ using namespace lu2mi_tiles;
 using namespace lu2mi_gemm;
 
 // Narrow products (M or N <= 64: the recursion nodes near the leaves, their
-// TRSM and inverse blocks) in 64x64 tiles, which waste less of the matrix
-// cores on padding; every element keeps the K range of the 128x128 tile
-// that contains it, so the results are the 128x128 kernel's bits.
+// TRSM and inverse blocks) and products too small to fill the device in
+// 128x128 tiles run in 64x64 tiles, which waste less of the matrix cores on
+// padding and spread over more workgroups; every element keeps the K range
+// of the 128x128 tile that contains it, so the results are the 128x128
+// kernel's bits.
 template <typename T, int EPI>
 __global__ __launch_bounds__(NT) void gemm64_kernel(GemmDesc g, int tiles_m, int tiles_n)
 {
@@ -80,7 +82,16 @@ static int launch_gemm(const GemmDesc &g, hipStream_t s)
                      (g.sA % EPV == 0) && (g.sB % EPV == 0);
     const int epi = (g.alpha == -1.0 && g.beta == 1.0) ? EPI_SUB : (g.alpha == 1.0 && g.beta == 0.0) ? EPI_SET
                                                                                                    : EPI_GENERAL;
-    if (vec && g.tag == 0 && (g.M <= BT64 || g.N <= BT64)) {
+    // 64x64 tiles for narrow products, and for any other product whose
+    // 128x128 grid would not fill the device (two such workgroups per CU)
+    static int num_cu = 0;
+    if (!num_cu) {
+        int dev = 0;
+        LU2MI_CHECK(hipGetDevice(&dev));
+        LU2MI_CHECK(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const bool underfilled = total * g.batch < 2L * num_cu;
+    if (vec && g.tag == 0 && (g.M <= BT64 || g.N <= BT64 || underfilled)) {
         const int tm64 = (g.M + BT64 - 1) / BT64, tn64 = (g.N + BT64 - 1) / BT64;
         dim3 grid64((unsigned)(tm64 * tn64), (unsigned)g.batch), block64(NT);
         if (epi == EPI_SUB)

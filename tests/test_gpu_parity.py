@@ -101,13 +101,16 @@ def test_gemm_strided_submatrix(ctx):
     assert np.array_equal(got[:300], big[:300])
 
 
-@pytest.mark.parametrize("m,n,k,narrow", [(1000, 64, 300, "n"), (1000, 16, 48, "n"), (64, 900, 200, "m"),
-                                           (40, 700, 130, "m")])
-def test_gemm_narrow_tiles_bit_identical(ctx, m, n, k, narrow):
-    """Products with M or N <= 64 run in 64x64 tiles; each element gets the
-    same MFMA sequence as in the 128x128 tile of a wider product."""
+@pytest.mark.parametrize("m,n,k", [(1000, 64, 300), (1000, 16, 48), (64, 900, 200), (40, 700, 130),
+                                   (3000, 128, 128), (600, 256, 96)])
+def test_gemm_narrow_tiles_bit_identical(ctx, m, n, k):
+    """Products with M or N <= 64, or too small to fill the device in 128x128
+    tiles, run in 64x64 tiles; each element gets the same MFMA sequence as in
+    the 128x128 tile of a product wide enough for that kernel."""
     rng = np.random.default_rng(m + n + k)
-    wide_m, wide_n = (m, 200) if narrow == "n" else (200, n)
+    # the wide product fills at least 512 tiles of 128x128 (128 x 5 and more)
+    # and keeps 16-byte rows (the vector path)
+    wide_m, wide_n = max(m, 16384), max(n, 5 * 128)
     a = rng.standard_normal((wide_m, k))
     b = rng.standard_normal((k, wide_n))
     c = rng.standard_normal((wide_m, wide_n))
