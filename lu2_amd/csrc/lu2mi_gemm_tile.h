@@ -150,9 +150,12 @@ struct Tile64Smem {
     T Bs[2][Mfma<T>::BK * BT64];
 };
 
+// bscale (optional): row k of B is multiplied by bscale[k] as it is loaded,
+// one rounding, as a separately scaled copy of B would hold it
 template <typename T, bool EDGE, int EPI>
 __device__ __forceinline__ void gemm_tile64(const GemmDesc &g, const T *__restrict__ A, const T *__restrict__ B,
-                                            T *__restrict__ C, int m0, int n0, int K, Tile64Smem<T> &sm)
+                                            T *__restrict__ C, int m0, int n0, int K, Tile64Smem<T> &sm,
+                                            const T *__restrict__ bscale = nullptr)
 {
     typedef Mfma<T> F;
     typedef typename F::vec_t V;
@@ -176,14 +179,22 @@ __device__ __forceinline__ void gemm_tile64(const GemmDesc &g, const T *__restri
             if (!EDGE) {
                 ra[it] = *(const V *)(A + (long)(m0 + ar) * lda + k0 + ac);
                 rb[it] = *(const V *)(B + (long)(k0 + br) * ldb + n0 + bc);
+                if (bscale) {
+                    T *pb = (T *)&rb[it];
+                    const T sc = bscale[k0 + br];
+#pragma unroll
+                    for (int e = 0; e < EPV; e++) pb[e] = pb[e] * sc;
+                }
             } else {
                 T va[EPV], vb[EPV];
                 const int gr = m0 + ar, gk = k0 + br;
+                const T sc = (bscale && gk < K) ? bscale[gk] : T(1);
 #pragma unroll
                 for (int e = 0; e < EPV; e++) {
                     const int gc = k0 + ac + e, gn = n0 + bc + e;
                     va[e] = (gr < M && gc < K) ? A[(long)gr * lda + gc] : T(0);
-                    vb[e] = (gk < K && gn < N) ? B[(long)gk * ldb + gn] : T(0);
+                    vb[e] = (gk < K && gn < N) ? (bscale ? B[(long)gk * ldb + gn] * sc : B[(long)gk * ldb + gn])
+                                               : T(0);
                 }
                 ra[it] = *(V *)va;
                 rb[it] = *(V *)vb;

@@ -732,6 +732,44 @@ __global__ __launch_bounds__(NTL, 2) void subpanel_kernel(SubArgs<T> s)
 // per node before its right half, two per node after it
 __host__ __device__ constexpr int sub_barriers(int L) { return 2 * L + 5 * (L - 1); }
 
+// ---------------------------------------------------------------------------
+// Recursion nodes of half-width h <= 64, one workgroup (one 64x64 tile) per
+// step instead of two launches each; same arithmetic as the separate steps.
+// node_trsm64: X = inv(D^-1 L11) * (rows [c, c+h) of the right half scaled
+// by 1/d), the scaling done as the rows are loaded (the staging copy's
+// values); the tile reads all of its rows before it writes them back.
+template <typename T>
+__global__ __launch_bounds__(256) void node_trsm64_kernel(T *A, long lda, int c, int h, const T *linv, int ldl,
+                                                          long off, const T *invd)
+{
+    using namespace lu2mi_gemm;
+    __shared__ Tile64Smem<T> sm;
+    GemmDesc g;
+    g.M = h; g.N = h; g.K = h; g.alpha = 1.0; g.beta = 0.0;
+    g.lda = ldl; g.ldb = lda; g.ldc = lda; g.a_lower = 1;
+    T *X = A + (long)c * lda + c + h;
+    gemm_tile64<T, true, EPI_SET>(g, linv + off * ldl + off, X, X, 0, 0, h, sm, invd + c);
+}
+
+// node_inv64: the node's off-diagonal inverse block, X21 = -inv22 * (L21 * inv11),
+// both products in one workgroup (tmp passes between them through memory)
+template <typename T>
+__global__ __launch_bounds__(256) void node_inv64_kernel(const T *A, long lda, int c, int h, T *linv, int ldl,
+                                                         long off, T *tmp)
+{
+    using namespace lu2mi_gemm;
+    __shared__ Tile64Smem<T> sm;
+    GemmDesc g;
+    g.M = h; g.N = h; g.K = h; g.alpha = 1.0; g.beta = 0.0;
+    g.lda = lda; g.ldb = ldl; g.ldc = h; g.a_lower = 0;
+    gemm_tile64<T, true, EPI_SET>(g, A + (long)(c + h) * lda + c, linv + off * ldl + off, tmp, 0, 0, h, sm);
+    __threadfence();
+    __syncthreads();
+    g.alpha = -1.0; g.lda = ldl; g.ldb = h; g.ldc = ldl; g.a_lower = 1;
+    gemm_tile64<T, true, EPI_GENERAL>(g, linv + (off + h) * ldl + off + h, tmp, linv + (off + h) * ldl + off, 0, 0,
+                                      h, sm);
+}
+
 // column-major panel (m columns of ld rows, rows [r0, r0+R)) <-> row-major (R x ldp)
 __global__ void cm_to_rm_kernel(const double *cm, long ld, int r0, int R, int m, double *rm, long ldp, int back)
 {
@@ -1222,6 +1260,19 @@ struct Factor {
         if (w <= W) return leaf(i, c);
         const int h = w / 2;
         LU2MI_TRY(panel(i, c, h));
+        const long off = (long)(c - i);
+        if (h <= lu2mi_gemm::BT64) {   // one workgroup per step (see node_trsm64_kernel)
+            hipLaunchKernelGGL(node_trsm64_kernel<T>, dim3(1), dim3(256), 0, ctx->stream, A, lda, c, h, linv, nb, off,
+                               Prec<T>::invd(ctx));
+            LU2MI_CHECK(hipGetLastError());
+            LU2MI_TRY(gemm_call<T>(ctx, n - c - h, h, h, -1.0, A + (long)(c + h) * lda + c, lda,
+                                   A + (long)c * lda + c + h, lda, 1.0, A + (long)(c + h) * lda + c + h, lda, 0));
+            LU2MI_TRY(panel(i, c + h, h));
+            hipLaunchKernelGGL(node_inv64_kernel<T>, dim3(1), dim3(256), 0, ctx->stream, A, lda, c, h, linv, nb, off,
+                               tmp);
+            LU2MI_CHECK(hipGetLastError());
+            return LU2MI_OK;
+        }
         // right half of this node: rows [c, c+h) scaled by 1/d, then
         // X = inv(D^-1 L11) * scaled rows, then the rows below
         {
@@ -1229,7 +1280,6 @@ struct Factor {
             hipLaunchKernelGGL(scale_copy_kernel<T>, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, ctx->stream,
                                A, lda, c, c + h, h, h, Prec<T>::invd(ctx), pwbuf);
             LU2MI_CHECK(hipGetLastError());
-            const long off = (long)(c - i);
             LU2MI_TRY(gemm_call<T>(ctx, h, h, h, 1.0, linv + off * nb + off, nb, pwbuf, h, 0.0,
                                    A + (long)c * lda + c + h, lda, 1));
             LU2MI_TRY(gemm_call<T>(ctx, n - c - h, h, h, -1.0, A + (long)(c + h) * lda + c, lda,
@@ -1238,7 +1288,6 @@ struct Factor {
         LU2MI_TRY(panel(i, c + h, h));
         // off-diagonal block of the inverse: X21 = -inv22 * L21 * inv11
         {
-            const long off = (long)(c - i);
             LU2MI_TRY(gemm_call<T>(ctx, h, h, h, 1.0, A + (long)(c + h) * lda + c, lda, linv + off * nb + off, nb,
                                    0.0, tmp, h, 0));
             LU2MI_TRY(gemm_call<T>(ctx, h, h, h, -1.0, linv + (off + h) * nb + off + h, nb, tmp, h, 0.0,
