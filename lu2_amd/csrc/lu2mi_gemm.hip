@@ -83,16 +83,16 @@ static int launch_gemm(const GemmDesc &g, hipStream_t s)
     const int epi = (g.alpha == -1.0 && g.beta == 1.0) ? EPI_SUB : (g.alpha == 1.0 && g.beta == 0.0) ? EPI_SET
                                                                                                    : EPI_GENERAL;
     // 64x64 tiles for narrow products, and for any other product whose
-    // 128x128 grid would not give every CU four such workgroups (two rounds
-    // at two per CU; measured 13 ms faster per n=65536 factorization than
-    // a one-round threshold)
+    // 128x128 grid would not give every CU eight such workgroups (four rounds
+    // at two per CU; per n=65536 factorization 13 ms faster with a two-round
+    // threshold than with one, and another ~10 ms with four)
     static int num_cu = 0;
     if (!num_cu) {
         int dev = 0;
         LU2MI_CHECK(hipGetDevice(&dev));
         LU2MI_CHECK(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    const bool underfilled = total * g.batch < 4L * num_cu;
+    const bool underfilled = total * g.batch < 8L * num_cu;
     if (vec && g.tag == 0 && (g.M <= BT64 || g.N <= BT64 || underfilled)) {
         const int tm64 = (g.M + BT64 - 1) / BT64, tn64 = (g.N + BT64 - 1) / BT64;
         dim3 grid64((unsigned)(tm64 * tn64), (unsigned)g.batch), block64(NT);

@@ -108,9 +108,9 @@ def test_gemm_narrow_tiles_bit_identical(ctx, m, n, k):
     tiles, run in 64x64 tiles; each element gets the same MFMA sequence as in
     the 128x128 tile of a product wide enough for that kernel."""
     rng = np.random.default_rng(m + n + k)
-    # the wide product has at least 1024 tiles of 128x128 (128 x 9 and more)
+    # the wide product has at least 2048 tiles of 128x128 (128 x 17 and more)
     # and keeps 16-byte rows (the vector path)
-    wide_m, wide_n = max(m, 16384), max(n, 9 * 128)
+    wide_m, wide_n = max(m, 16384), max(n, 17 * 128)
     a = rng.standard_normal((wide_m, k))
     b = rng.standard_normal((k, wide_n))
     c = rng.standard_normal((wide_m, wide_n))
