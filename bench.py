@@ -48,6 +48,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix, published
+TRAIL_KERNEL_NAME = {1: "gemm2_kernel<1> (trailing update A22 -= L21*U12, operands loaded straight into LDS)",
+                     0: "gemm_kernel<double, true, 1, 1> (trailing update A22 -= L21*U12, register-staged)"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -156,6 +158,44 @@ def batched_config(ctx, seed: int) -> dict:
     return out
 
 
+def gemm_probe(m: int, n: int, k: int, reps: int, trail_kernel: int) -> int:
+    """The trailing-update product on its own: C[m x n] -= A[m x k] B[k x n]
+    inside one row-major matrix (A = rows k.., columns 0..k; B = rows 0..k,
+    columns k..; C = rows k.., columns k..), as the factorization lays it
+    out, with the n=65536 factorization's leading dimension 65552 (or
+    m + k + 16 if larger), after a 1 GiB device-to-device copy that
+    calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters.  Prints one JSON
+    line with the mean time and rate over `reps` launches."""
+    import lu2_amd as L
+
+    lda = max(m + k + 16, 65552)
+    ctx = L.Context(0)
+    ctx.set_option(L.OPT_TRAIL_KERNEL, trail_kernel)
+    ctx.set_option(L.OPT_DGEMM_TRAILING, 1)
+    dX = ctx.alloc(8 * (m + k) * lda)
+    ctx.randomsetmat(m + k, 1, dX, lda)
+    cal = ctx.alloc(1 << 30)
+    tmp = ctx.alloc(1 << 30)
+    tmp.copy_from(cal)                       # the calibration copy
+    offA, offB, offC = 8 * k * lda, 8 * k, 8 * (k * lda + k)
+    times = []
+    for _ in range(reps):
+        L._lib.lu2mi_synchronize(ctx.handle)
+        t0 = time.perf_counter()
+        ctx.dgemm_dev(m, n, k, -1.0, dX, lda, dX, lda, 1.0, dX, lda, offs=(offA, offB, offC))
+        times.append(time.perf_counter() - t0)
+    for b in (dX, cal, tmp):
+        b.free()
+    ctx.close()
+    t = sorted(times)[len(times) // 2]
+    fl = 2.0 * m * n * k
+    print(json.dumps({"probe": "trailing-update DGEMM", "kernel": TRAIL_KERNEL_NAME[trail_kernel], "M": m, "N": n, "K": k, "lda": lda, "reps": reps,
+                      "median_ms": t * 1e3, "tflops": fl / t / 1e12,
+                      "frac_of_fp64_peak": fl / t / 1e12 / FP64_MFMA_PEAK_TFLOPS,
+                      "algorithmic_bytes": 8.0 * (2 * m * n + m * k + k * n)}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,8 +221,15 @@ def main() -> int:
     ap.add_argument("--subpanel", type=int, default=None,
                     help="widest panel part factored by one persistent launch (0 = one launch per "
                          "step; library default if omitted)")
+    ap.add_argument("--gemm-probe", type=int, nargs=3, metavar=("M", "N", "K"), default=None,
+                    help="time the trailing-update DGEMM alone at this shape (counter runs)")
+    ap.add_argument("--probe-reps", type=int, default=1)
+    ap.add_argument("--trail-kernel", type=int, default=0, choices=(0, 1),
+                    help="trailing-update DGEMM tile: 1 direct-to-LDS (default), 0 register-staged")
     args = ap.parse_args()
 
+    if args.gemm_probe:
+        return gemm_probe(*args.gemm_probe, args.probe_reps, args.trail_kernel)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the LU on a single MI355X "
               "(no multi-GPU path); run with --gpus 1", file=sys.stderr)
@@ -201,6 +248,7 @@ def main() -> int:
     ctx.set_option(L.OPT_PIVMODE, piv_mode)
     ctx.set_option(L.OPT_LOOKAHEAD, args.lookahead)
     ctx.set_option(L.OPT_LEAF_W, args.leaf_w)
+    ctx.set_option(L.OPT_TRAIL_KERNEL, args.trail_kernel)
     if args.subpanel is not None:
         ctx.set_option(L.OPT_SUBPANEL, args.subpanel)
     nbytes = 8 * n * lda
@@ -291,13 +339,14 @@ def main() -> int:
         "config": {"workload": f"lu2 FP64 LU with partial pivoting + solve, n={n}, nb={nb}, "
                                f"pivot={args.pivot}, 1 RHS, seed {args.seed}",
                    "n": n, "nb": nb, "pivot": args.pivot, "lookahead": args.lookahead, "leaf_w": args.leaf_w,
-                   "subpanel": args.subpanel if args.subpanel is not None else 0},
+                   "subpanel": args.subpanel if args.subpanel is not None else 0,
+                   "trail_kernel": args.trail_kernel},
         "tflops_2n3_3_plus_2n2": (flops + 2.0 * n * n) / mean / 1e12,
         "frac_of_fp64_peak": flops / mean / 1e12 / FP64_MFMA_PEAK_TFLOPS,
         "hpl2_residual": float(res[6]),
         "roofline": {
             "bound": "mfma",
-            "kernel": "gemm_kernel<double, true, 1, 1> (trailing update A22 -= L21*U12)",
+            "kernel": TRAIL_KERNEL_NAME[args.trail_kernel],
             "achieved": achieved,
             "peak": FP64_MFMA_PEAK_TFLOPS,
             "unit": "TFLOP/s",

@@ -38,6 +38,8 @@ OPT_LEAF_W = 2
 OPT_TIMING = 3
 OPT_LOOKAHEAD = 4
 OPT_SUBPANEL = 5
+OPT_TRAIL_KERNEL = 6
+OPT_DGEMM_TRAILING = 7
 BATCH_EAGER = 1
 BATCH_NMAX = 512
 BATCH_RHSMAX = 8

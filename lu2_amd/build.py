@@ -35,13 +35,14 @@ def _run(cmd, **kw):
 
 
 def build_library() -> str:
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(CSRC, "_build", src.replace(".hip", ".o"))
-        os.makedirs(os.path.dirname(obj), exist_ok=True)
-        _run([HIPCC, *HIPFLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-              "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+    from concurrent.futures import ThreadPoolExecutor
+
+    objs = [os.path.join(CSRC, "_build", src.replace(".hip", ".o")) for src in SOURCES]
+    os.makedirs(os.path.join(CSRC, "_build"), exist_ok=True)
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: _run([HIPCC, *HIPFLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+                                     "-c", os.path.join(CSRC, so[0]), "-o", so[1]]), zip(SOURCES, objs)))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
@@ -55,8 +56,15 @@ def build_driver() -> str:
     return DRIVER
 
 
+REFERENCE = os.environ.get("LU2_REFERENCE", "/root/reference")
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    # lu2's own driver from the reference sources, linked against the library
+    # (only where the reference is present; the built binary travels)
+    if os.path.exists(os.path.join(REFERENCE, "lu2.c")):
+        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REFERENCE}"])
 
 
 def build_all() -> None:

@@ -58,6 +58,8 @@ extern "C" int lu2mi_init(int device, lu2mi_ctx **out)
     LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_next, hipEventDisableTiming));
     LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming));
     ctx->lookahead = 1;
+    ctx->trail_kernel = 0;
+    ctx->dgemm_as_trailing = 0;
     ctx->sub_w = 0;
     LU2MI_CHECK(hipMalloc(&ctx->d_bar, 256));
     LU2MI_CHECK(hipMemset(ctx->d_bar, 0, 256));
@@ -120,6 +122,13 @@ extern "C" int lu2mi_set_option(lu2mi_ctx *ctx, int opt, int value)
             return LU2MI_ERR_ARG;
         }
         ctx->sub_w = value;
+        return LU2MI_OK;
+    case LU2MI_OPT_TRAIL_KERNEL:
+        if (value != 0 && value != 1) return LU2MI_ERR_ARG;
+        ctx->trail_kernel = value;
+        return LU2MI_OK;
+    case LU2MI_OPT_DGEMM_TRAILING:
+        ctx->dgemm_as_trailing = value ? 1 : 0;
         return LU2MI_OK;
     case LU2MI_OPT_TIMING:
         ctx->timing_on = value ? 1 : 0;
@@ -230,6 +239,7 @@ extern "C" int lu2mi_dgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, double alpha
     g.M = m; g.N = n; g.K = k; g.batch = 1; g.alpha = alpha; g.beta = beta;
     g.A = dA; g.lda = lda; g.sA = 0; g.B = dB; g.ldb = ldb; g.sB = 0; g.C = dC; g.ldc = ldc; g.sC = 0;
     g.a_lower = 0;
+    if (ctx->dgemm_as_trailing) g.tag = ctx->trail_kernel ? 1 : 2;
     LU2MI_TRY(lu2mi_gemm_f64(g, ctx->stream));
     LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
     return LU2MI_OK;

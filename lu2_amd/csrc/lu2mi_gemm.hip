@@ -58,6 +58,26 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmDesc g, int tiles_m, int t
     else gemm_tile<T, true, EPI>(g, A, B, C, m0, n0, K, As, Bs);
 }
 
+// The trailing update (tag 1, FP64) in the direct-to-LDS tile of
+// lu2mi_gemm2; edge tiles run the same arithmetic with bounds-checked loads.
+template <int EPI>
+__global__ __launch_bounds__(lu2mi_gemm2::NT, 2) void gemm2_kernel(GemmDesc g, int tiles_m, int tiles_n)
+{
+    namespace G2 = lu2mi_gemm2;
+    __shared__ G2::Smem sm;
+    const int total = tiles_m * tiles_n;
+    const int tile_id = xcd_remap(blockIdx.x, total);
+    int tm, tn;
+    tile_coords(tile_id, tiles_m, tiles_n, tm, tn);
+    const int m0 = tm * G2::BM, n0 = tn * G2::BN;
+    const double *A = (const double *)g.A;
+    const double *B = (const double *)g.B;
+    double *C = (double *)g.C;
+    const bool interior = (m0 + G2::BM <= g.M) && (n0 + G2::BN <= g.N) && (g.K % G2::BK == 0);
+    if (interior) G2::tile<false, EPI>(g, A, B, C, m0, n0, g.K, sm);
+    else G2::tile<true, EPI>(g, A, B, C, m0, n0, g.K, sm);
+}
+
 // ---------------------------------------------------------------------------
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
@@ -106,7 +126,13 @@ static int launch_gemm(const GemmDesc &g, hipStream_t s)
         return LU2MI_OK;
     }
     dim3 grid((unsigned)total, (unsigned)g.batch), block(NT);
-    if (vec && epi == EPI_SUB && g.tag == 1)
+    if (sizeof(T) == 8 && g.tag == 1 && epi == EPI_SUB && vec && g.batch == 1 &&
+        !g.a_lower && aligned16(g.C) && g.ldc % 2 == 0) {
+        hipLaunchKernelGGL(gemm2_kernel<EPI_SUB>, grid, block, 0, s, g, tiles_m, tiles_n);
+        LU2MI_CHECK(hipGetLastError());
+        return LU2MI_OK;
+    }
+    if (vec && epi == EPI_SUB && g.tag != 0)
         hipLaunchKernelGGL((gemm_kernel<T, true, EPI_SUB, 1>), grid, block, 0, s, g, tiles_m, tiles_n);
     else if (vec && epi == EPI_SUB)
         hipLaunchKernelGGL((gemm_kernel<T, true, EPI_SUB, 0>), grid, block, 0, s, g, tiles_m, tiles_n);

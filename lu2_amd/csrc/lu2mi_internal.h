@@ -35,7 +35,10 @@ struct GemmDesc {
     const void *B = nullptr; long ldb = 0; long sB = 0;
     void *C = nullptr; long ldc = 0; long sC = 0;
     int a_lower = 0;
-    int tag = 0;   // 1: trailing update (separate kernel symbol for profiling)
+    // 0: any product; 1: trailing update in the direct-to-LDS FP64 tile
+    // (gemm2_kernel); 2: trailing update in the register-staged tile.  Both
+    // trailing forms are kernel symbols of their own (for profiling).
+    int tag = 0;
 };
 int lu2mi_gemm_f64(const GemmDesc &g, hipStream_t s);
 int lu2mi_gemm_f32(const GemmDesc &g, hipStream_t s);
@@ -77,6 +80,8 @@ struct lu2mi_ctx {
     int leaf_w;
     int timing_on;
     int lookahead;         // factor panel i+1 on stream2 during block i's trailing update
+    int trail_kernel;      // trailing-update tile: 1 direct-to-LDS FP64, 0 register-staged
+    int dgemm_as_trailing; // lu2mi_dgemm_dev runs its product as a trailing update
     hipStream_t stream2;
     hipEvent_t ev_next, ev_panel;
     KernelTimer tgemm;

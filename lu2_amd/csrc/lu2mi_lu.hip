@@ -1329,7 +1329,7 @@ struct Factor {
                                      (float *)(A + (long)(i + nb) * lda + c0), lda));
         else
             LU2MI_TRY(gemm_call<T>(ctx, M, N, nb, -1.0, A + (long)(i + nb) * lda + i, lda, A + (long)i * lda + c0,
-                                   lda, 1.0, A + (long)(i + nb) * lda + c0, lda, 0, 1));
+                                   lda, 1.0, A + (long)(i + nb) * lda + c0, lda, 0, ctx->trail_kernel ? 1 : 2));
         if (tm) {
             LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used + 1], ctx->stream));
             kt.flops[kt.used] = 2.0 * M * (double)N * nb;

@@ -13,7 +13,10 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "_build", "liblu2oracle.so")
+# LU2O_LIB selects another build of the same source, e.g. the sanitizer
+# build (make -C oracle asan: _build/liblu2oracle_asan.so, run with libasan
+# preloaded)
+LIB = os.environ.get("LU2O_LIB") or os.path.join(HERE, "_build", "liblu2oracle.so")
 PIV_COMPAT = 0
 PIV_GEPP = 1
 

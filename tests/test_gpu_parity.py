@@ -128,6 +128,66 @@ def test_gemm_narrow_tiles_bit_identical(ctx, m, n, k):
     assert np.abs(out["narrow"] - (c[:m, :n] - a[:m] @ b[:, :n])).max() < 1e-12 * k
 
 
+def trailing_product(ctx, a, b, c, kernel):
+    """C - A B through the trailing-update kernel (even leading dimensions,
+    16-byte aligned rows: the shape the factorization hands it)."""
+    m, k = a.shape
+    n = b.shape[1]
+    lda, ldb = k + (k & 1), n + (n & 1)
+    ap = np.zeros((m, lda)); ap[:, :k] = a
+    bp = np.zeros((k, ldb)); bp[:, :n] = b
+    cp = np.zeros((m, ldb)); cp[:, :n] = c
+    dA, dB, dC = ctx.alloc(ap.nbytes), ctx.alloc(bp.nbytes), ctx.alloc(cp.nbytes)
+    dA.upload(ap)
+    dB.upload(bp)
+    dC.upload(cp)
+    ctx.set_option(L.OPT_TRAIL_KERNEL, kernel)
+    ctx.set_option(L.OPT_DGEMM_TRAILING, 1)
+    try:
+        ctx.dgemm_dev(m, n, k, -1.0, dA, lda, dB, ldb, 1.0, dC, ldb)
+    finally:
+        ctx.set_option(L.OPT_DGEMM_TRAILING, 0)
+        ctx.set_option(L.OPT_TRAIL_KERNEL, 0)   # the library default
+    out = dC.download(np.float64, m * ldb).reshape(m, ldb)
+    for d in (dA, dB, dC):
+        d.free()
+    assert np.array_equal(out[:, n:], cp[:, n:])      # nothing written past column n
+    return out[:, :n]
+
+
+@pytest.mark.parametrize("kernel", [1, 0])
+@pytest.mark.parametrize("m,n,k", [(128, 128, 16), (300, 257, 512), (1000, 999, 2048), (256, 384, 32),
+                                   (640, 129, 48), (129, 1, 64), (1, 300, 16), (384, 256, 40)])
+def test_trailing_gemm_error_bound(ctx, kernel, m, n, k):
+    """The trailing-update kernels (operands loaded straight into LDS, or
+    staged through registers) against the componentwise bound
+    |C - fl(C - AB)| <= gamma_{K+2} (|A||B| + |C|), edge tiles (M, N not
+    multiples of 128, odd N, K not a multiple of 16) included."""
+    rng = np.random.default_rng(m * 31 + n * 7 + k)
+    a, b, c = rng.standard_normal((m, k)), rng.standard_normal((k, n)), rng.standard_normal((m, n))
+    got = trailing_product(ctx, a, b, c, kernel)
+    ref = c - a @ b
+    u = 2.0 ** -53
+    gamma = (k + 2) * u / (1 - (k + 2) * u)
+    bound = 2 * gamma * (np.abs(a) @ np.abs(b) + np.abs(c))
+    assert np.all(np.abs(got - ref) <= bound + 1e-300)
+
+
+def test_trailing_gemm_edge_tiles_bit_identical(ctx):
+    """Edge tiles (bounds-checked loads through registers) perform exactly
+    the arithmetic of interior tiles (loads straight into LDS): the 300 x 257
+    product equals the corresponding corner of a 512 x 512 product, whose
+    tiles are all interior, bit for bit."""
+    rng = np.random.default_rng(11)
+    k = 96
+    a = rng.standard_normal((512, k))
+    b = rng.standard_normal((k, 512))
+    c = rng.standard_normal((512, 512))
+    big = trailing_product(ctx, a, b, c, 1)
+    small = trailing_product(ctx, a[:300], b[:, :257], c[:300, :257], 1)
+    assert np.array_equal(big[:300, :257], small)
+
+
 # ----------------------------------------------------------------------- LU
 # (256, 256): one block; (512, 256): two blocks, so the split schedule's
 # remaining columns are b alone; (64, 32): the smallest block

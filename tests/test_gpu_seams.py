@@ -147,7 +147,8 @@ def test_lumcolumn_host_dropin(lib, oracle, n, nb, seed):
     assert np.array_equal(a[:, : n + 1], fd[:, : n + 1])
 
 
-@pytest.mark.parametrize("n,m,i", [(300, 16, 0), (1000, 16, 64), (777, 32, 100)])
+# more than 32768 rows: the leaf holds four rows per thread
+@pytest.mark.parametrize("n,m,i", [(300, 16, 0), (1000, 16, 64), (777, 32, 100), (40000, 16, 0), (41000, 16, 1000)])
 def test_cm_column_decomposition_recursive(lib, oracle, n, m, i):
     """lu2tlib.c:1130 on the column-major awork of lu2.c:1072-1085: same
     pivots as the oracle's restatement, values to rounding (lu2's rule)."""
@@ -175,3 +176,43 @@ def test_driver_lu2mi(oracle):
     out = r.stdout
     assert "N=2048 Seed=1 NB=256" in out
     assert "Emax=" in out and "Gflops" in out and "PASSED" in out
+
+
+DROPIN = os.path.join(ROOT, "oracle", "_ref", "lu2_dropin")
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN),
+                    reason="lu2's own driver is compiled from the reference sources (make -C oracle ref), "
+                           "which were not available where this tree was built")
+@pytest.mark.parametrize("pivot", ["gepp", "compat"])
+def test_reference_driver_dropin(oracle, pivot):
+    """lu2.c's own main (lu2.c:1220-1309) with lu2_mp_nompi.c, timerlib.c and
+    second.c, compiled unmodified and linked against liblu2mi.so (no CBLAS):
+    its panel leaves (cm_column_decomposition_recursive), TRSM and trailing
+    products (mydgemm, matmul_for_small_nk) and reset_gdr all run through the
+    library's seams on host memory.  With the GEPP leaf rule its Emax
+    (lu2.c:45-60 on the regenerated matrix) is that of a backward-stable
+    solve; with lu2's own rule it is within the spread that rule shows
+    between two correct implementations."""
+    import re
+
+    n, nb = 2048, 256
+    env = dict(os.environ, LU2MI_PIVOT=pivot, OMP_NUM_THREADS="8")
+    r = subprocess.run([DROPIN, "-n", str(n), "-b", str(nb), "-s", "1"], capture_output=True, text=True,
+                       timeout=600, env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = r.stdout
+    assert f"N={n} Seed=1 NB={nb}" in out
+    m = re.search(r"Emax=\s*(\S+)", out)
+    assert m, out
+    emax = float(m.group(1))
+    assert re.search(r"Nswap=\d+ cpsec =\s+\S+ wsec=\S+ \S+ Gflops", out), out
+    for name in ("swaprows", "DGEMM2K", "Total"):
+        assert name in out
+    a0 = oracle.randomsetmat(n, 1)
+    xo, _, _ = oracle.lumcolumn(a0, nb, oracle.PIV_GEPP if pivot == "gepp" else oracle.PIV_COMPAT)
+    eo = oracle.emax(a0, xo)
+    if pivot == "gepp":
+        assert emax <= 1e-9 and emax <= 100 * eo, (emax, eo)
+    else:
+        assert np.isfinite(emax) and emax <= 1e4 * eo, (emax, eo)
