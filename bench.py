@@ -221,6 +221,8 @@ def main() -> int:
     ap.add_argument("--subpanel", type=int, default=None,
                     help="widest panel part factored by one persistent launch (0 = one launch per "
                          "step; library default if omitted)")
+    ap.add_argument("--strict-sync", type=int, default=0, choices=(0, 1),
+                    help="leaf and back substitution publish with agent-scope release/acquire")
     ap.add_argument("--gemm-probe", type=int, nargs=3, metavar=("M", "N", "K"), default=None,
                     help="time the trailing-update DGEMM alone at this shape (counter runs)")
     ap.add_argument("--probe-reps", type=int, default=1)
@@ -249,6 +251,7 @@ def main() -> int:
     ctx.set_option(L.OPT_LOOKAHEAD, args.lookahead)
     ctx.set_option(L.OPT_LEAF_W, args.leaf_w)
     ctx.set_option(L.OPT_TRAIL_KERNEL, args.trail_kernel)
+    ctx.set_option(L.OPT_STRICT_SYNC, args.strict_sync)
     if args.subpanel is not None:
         ctx.set_option(L.OPT_SUBPANEL, args.subpanel)
     nbytes = 8 * n * lda
@@ -340,7 +343,7 @@ def main() -> int:
                                f"pivot={args.pivot}, 1 RHS, seed {args.seed}",
                    "n": n, "nb": nb, "pivot": args.pivot, "lookahead": args.lookahead, "leaf_w": args.leaf_w,
                    "subpanel": args.subpanel if args.subpanel is not None else 0,
-                   "trail_kernel": args.trail_kernel},
+                   "trail_kernel": args.trail_kernel, "strict_sync": args.strict_sync},
         "tflops_2n3_3_plus_2n2": (flops + 2.0 * n * n) / mean / 1e12,
         "frac_of_fp64_peak": flops / mean / 1e12 / FP64_MFMA_PEAK_TFLOPS,
         "hpl2_residual": float(res[6]),

@@ -56,6 +56,11 @@ extern "C" {
                                      two sum each product's K terms in different orders */
 #define LU2MI_OPT_DGEMM_TRAILING 7 /* 1: lu2mi_dgemm_dev runs its product with the
                                      trailing-update kernel (to measure that kernel alone) */
+#define LU2MI_OPT_STRICT_SYNC 8   /* 1: the panel leaf and the back substitution publish their
+                                     cross-workgroup data with agent-scope release/acquire
+                                     (an L2 writeback and invalidate per publication); default
+                                     0: device-coherent stores completed in order (results
+                                     identical either way) */
 
 typedef struct lu2mi_ctx lu2mi_ctx;
 

@@ -40,6 +40,7 @@ OPT_LOOKAHEAD = 4
 OPT_SUBPANEL = 5
 OPT_TRAIL_KERNEL = 6
 OPT_DGEMM_TRAILING = 7
+OPT_STRICT_SYNC = 8
 BATCH_EAGER = 1
 BATCH_NMAX = 512
 BATCH_RHSMAX = 8
@@ -232,6 +233,9 @@ class Context:
         out = np.zeros(10)
         _check(_lib.lu2mi_residual_dev(self.handle, n, dA0.ptr, lda, dx.ptr, out.ctypes.data))
         return out
+
+    def synchronize(self) -> None:
+        _check(_lib.lu2mi_synchronize(self.handle))
 
     def timing(self) -> dict:
         out = np.zeros(4)

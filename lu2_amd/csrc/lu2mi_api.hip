@@ -130,6 +130,9 @@ extern "C" int lu2mi_set_option(lu2mi_ctx *ctx, int opt, int value)
     case LU2MI_OPT_DGEMM_TRAILING:
         ctx->dgemm_as_trailing = value ? 1 : 0;
         return LU2MI_OK;
+    case LU2MI_OPT_STRICT_SYNC:
+        ctx->strict_sync = value ? 1 : 0;
+        return LU2MI_OK;
     case LU2MI_OPT_TIMING:
         ctx->timing_on = value ? 1 : 0;
         if (ctx->timing_on && ctx->tgemm.cap == 0) {

@@ -82,6 +82,7 @@ struct lu2mi_ctx {
     int lookahead;         // factor panel i+1 on stream2 during block i's trailing update
     int trail_kernel;      // trailing-update tile: 1 direct-to-LDS FP64, 0 register-staged
     int dgemm_as_trailing; // lu2mi_dgemm_dev runs its product as a trailing update
+    int strict_sync;       // leaf and back substitution publish with release/acquire
     hipStream_t stream2;
     hipEvent_t ev_next, ev_panel;
     KernelTimer tgemm;

@@ -131,6 +131,32 @@ __device__ __forceinline__ void st_dev(float *p, float v)
     __hip_atomic_store((unsigned *)p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Publication of a sequence word (leaf slots, back-substitution frontier)
+// after the data it announces.  Every exchanged word is written and read with
+// device-coherent (agent-scope, sc1) accesses, so no cache maintenance is
+// needed, only completion order: the compiler barriers keep the data stores
+// ahead of the sequence store in program order and s_waitcnt vmcnt(0) makes
+// them complete before it is issued.  strict: an agent-scope release store
+// instead (on gfx950 it adds buffer_wbl2, an L2 writeback, per publication;
+// the matching acquire on the reader adds buffer_inv, an L2 invalidate).
+__device__ __forceinline__ void publish_seq(unsigned long long *p, unsigned long long v, int strict)
+{
+    if (strict) {
+        __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+// reader side, after the spin on a sequence word has seen the new value
+__device__ __forceinline__ void acquire_seq(int strict)
+{
+    if (strict) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // rank-1 update of one element: x - l * (u * s), with u*s rounded first (the
 // stored U value) and one fused multiply-subtract, written out so that every
 // copy of a row computes bit-identical values
@@ -164,6 +190,7 @@ struct LeafArgs {
     unsigned long long *slots;    // 2 * (G+1) * SLOT_WORDS, G <= LEAF_GMAX
     unsigned long long seq_base;  // step sequence numbers seq_base+1 .. seq_base+W
     int *err;
+    int strict;                   // release/acquire publication (publish_seq)
 };
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -280,14 +307,12 @@ __device__ __forceinline__ void leaf_body(const LeafArgs<T> &a, const int bid, L
                                        ((unsigned long long)(unsigned)borig[q] << 32));
 #pragma clang loop unroll(full)
                 for (int k = 0; k < W; k++) st_relaxed(sl + 3 + k, dbits((double)value(q, k)));
-                __builtin_amdgcn_s_waitcnt(0);
-                st_relaxed(sl + SLOT_WORDS - 1, seq);
+                publish_seq(sl + SLOT_WORDS - 1, seq, a.strict);
             }
         if (wr >= n && t == 0) {
             st_relaxed(sl + 0, dbits(-1.0));
             st_relaxed(sl + 1, (unsigned long long)(unsigned)INT_MAX);
-            __builtin_amdgcn_s_waitcnt(0);
-            st_relaxed(sl + SLOT_WORDS - 1, seq);
+            publish_seq(sl + SLOT_WORDS - 1, seq, a.strict);
         }
         if (bid == 0 && t == jc) {
             unsigned long long *cs = slots + ((long)par * (G + 1) + G) * SLOT_WORDS;
@@ -295,8 +320,7 @@ __device__ __forceinline__ void leaf_body(const LeafArgs<T> &a, const int bid, L
                                    ((unsigned long long)(unsigned)borig[0] << 32));
 #pragma clang loop unroll(full)
             for (int k = 0; k < W; k++) st_relaxed(cs + 3 + k, dbits((double)value(0, k)));
-            __builtin_amdgcn_s_waitcnt(0);
-            st_relaxed(cs + SLOT_WORDS - 1, seq);
+            publish_seq(cs + SLOT_WORDS - 1, seq, a.strict);
         }
     };
 
@@ -321,7 +345,7 @@ __device__ __forceinline__ void leaf_body(const LeafArgs<T> &a, const int bid, L
                     }
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            acquire_seq(a.strict);
             const int boff = (int)(((long)par * (G + 1) + t) * SLOT_WORDS * 8);
 #pragma clang loop unroll(full)
             for (int k = 0; k < SW; k++) {
@@ -821,7 +845,7 @@ __device__ __forceinline__ float lane_bcast(float v, int l)
 
 template <typename TF, typename TX>
 __global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, long lda, int n, TX *x,
-                                                      unsigned long long *front, int *err)
+                                                      unsigned long long *front, int *err, int strict)
 {
     __shared__ TF Ud[BSB][BSB + 1];
     __shared__ TX part[BSB];
@@ -878,7 +902,7 @@ __global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, 
             }
             __syncthreads();
             const int F = min(s_front, idx);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            acquire_seq(strict);
             accumulate(done, pa);            // the prefetched tile
             int d = done + 1;
             for (; d + 1 < F; d += 2) {      // blocks already solved: two tiles in flight
@@ -913,10 +937,14 @@ __global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, 
                 if (t < rr) xt -= (TX)Ud[t][rr] * xk;
             }
             if (t < nrb) st_dev(x + r0 + t, xt);
-            __builtin_amdgcn_s_waitcnt(0);
-            if (t == 0) st_relaxed(front, (unsigned long long)(idx + 1));
+            // the block's x all come from this one wave, whose s_waitcnt
+            // vmcnt(0) in publish_seq covers every lane's store
+            if (!strict && t == 0) publish_seq(front, (unsigned long long)(idx + 1), 0);
         }
         __syncthreads();
+        // strict: the workgroup barrier orders the other lanes' stores before
+        // thread 0's release store
+        if (strict && t == 0) publish_seq(front, (unsigned long long)(idx + 1), 1);
     }
 }
 
@@ -977,7 +1005,18 @@ __global__ __launch_bounds__(256) void resid_final_kernel(const double *A, long 
         __syncthreads();
     }
     if (t < 6) out[t] = sh[t][0];
+    if (t == 0) {
+        // the scaled forms: HPL-2 with eps = 2^-53, and lu2_mpi.c:2170-2229's
+        // resid1..3 with lu2's eps = 2e-16
+        const double eps = 1.1102230246251565e-16, eps_lu2 = 2e-16, dn = (double)n;
+        const double r = sh[0][0], ainf = sh[1][0], a1 = sh[2][0], xinf = sh[3][0], x1 = sh[4][0], bmax = sh[5][0];
+        out[6] = r / (eps * (ainf * xinf + bmax) * dn);
+        out[7] = r / (eps_lu2 * a1 * dn);
+        out[8] = r / (eps_lu2 * a1 * x1);
+        out[9] = r / (eps_lu2 * ainf * xinf * dn);
+    }
 }
+
 
 // ===========================================================================
 // host side
@@ -1080,7 +1119,7 @@ static int backsub(lu2mi_ctx *ctx, const TF *A, long lda, int n, TX *x)
     if (G < 1) G = 1;
     LU2MI_CHECK(hipMemsetAsync(ctx->d_bar, 0, sizeof(unsigned long long), ctx->stream));
     hipLaunchKernelGGL((backsub_kernel<TF, TX>), dim3(G), dim3(256), 0, ctx->stream, A, lda, n, x, ctx->d_bar,
-                       ctx->d_err);
+                       ctx->d_err, ctx->strict_sync);
     LU2MI_CHECK(hipGetLastError());
     return LU2MI_OK;
 }
@@ -1123,7 +1162,7 @@ struct Factor {
         la.pair_dst = pairs; la.pair_src = pairs + 2 * W;
         la.linv = linv; la.ldl = nb;
         la.slots = (unsigned long long *)ctx->d_slots;
-        la.seq_base = ctx->bar_count; la.err = ctx->d_err;
+        la.seq_base = ctx->bar_count; la.err = ctx->d_err; la.strict = ctx->strict_sync;
         void *args[] = {&la};
         const void *fn = nullptr;
         if (W == 32) fn = (rpt == 1) ? (const void *)leaf_kernel<T, 32, 1> : (const void *)leaf_kernel<T, 32, 2>;
@@ -1227,7 +1266,7 @@ struct Factor {
         la.pair_dst = pairs; la.pair_src = pairs + 2 * SUB_W;
         la.linv = linv; la.ldl = nb;
         la.slots = (unsigned long long *)ctx->d_slots;
-        la.seq_base = ctx->bar_count; la.err = ctx->d_err;
+        la.seq_base = ctx->bar_count; la.err = ctx->d_err; la.strict = ctx->strict_sync;
         sa.nb = nb; sa.w = w;
         // strips of at least 64 bytes per row, about one per workgroup
         const int seg = 64 / (int)sizeof(T);
@@ -1553,6 +1592,20 @@ int lu2mi_diag_dominant_impl(lu2mi_ctx *ctx, int n, double *dA, long lda)
     return LU2MI_OK;
 }
 
+// the three residual kernels on ctx's stream: buf holds 3n doubles of
+// partial results, dout (device) receives the ten figures
+static int residual_launch(lu2mi_ctx *ctx, int n, const double *dA0, long lda, const double *dx, double *buf,
+                           double *dout)
+{
+    double *rerr = buf, *rabs = buf + n, *cabs = buf + 2 * (size_t)n;
+    hipLaunchKernelGGL(resid_rows_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, dA0, lda, n, dx, rerr, rabs);
+    hipLaunchKernelGGL(resid_cols_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, dA0, lda, n, cabs);
+    hipLaunchKernelGGL(resid_final_kernel, dim3(1), dim3(256), 0, ctx->stream, dA0, lda, n, dx, rerr, rabs, cabs,
+                       dout);
+    LU2MI_CHECK(hipGetLastError());
+    return LU2MI_OK;
+}
+
 int lu2mi_residual_impl(lu2mi_ctx *ctx, int n, const double *dA0, long lda, const double *dx, double *out)
 {
     if (!ctx || !dA0 || !dx || !out || n <= 0 || lda < n + 1) {
@@ -1560,25 +1613,18 @@ int lu2mi_residual_impl(lu2mi_ctx *ctx, int n, const double *dA0, long lda, cons
         return LU2MI_ERR_ARG;
     }
     double *buf = nullptr;
-    LU2MI_CHECK(hipMalloc(&buf, sizeof(double) * (3 * (size_t)n + 8)));
-    double *rerr = buf, *rabs = buf + n, *cabs = buf + 2 * (size_t)n, *dout = buf + 3 * (size_t)n;
-    hipLaunchKernelGGL(resid_rows_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, dA0, lda, n, dx, rerr, rabs);
-    hipLaunchKernelGGL(resid_cols_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, dA0, lda, n, cabs);
-    hipLaunchKernelGGL(resid_final_kernel, dim3(1), dim3(256), 0, ctx->stream, dA0, lda, n, dx, rerr, rabs, cabs,
-                       dout);
-    double h[6];
-    hipError_t e = hipMemcpyAsync(h, dout, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    LU2MI_CHECK(hipMalloc(&buf, sizeof(double) * (3 * (size_t)n + 16)));
+    int rc = residual_launch(ctx, n, dA0, lda, dx, buf, buf + 3 * (size_t)n);
+    hipError_t e = hipSuccess;
+    if (rc == LU2MI_OK) {
+        e = hipMemcpyAsync(out, buf + 3 * (size_t)n, 10 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
     (void)hipFree(buf);
+    if (rc != LU2MI_OK) return rc;
     if (e != hipSuccess) {
         lu2mi_set_error_msg("residual: %s", hipGetErrorString(e));
         return LU2MI_ERR_HIP;
     }
-    const double eps = 1.1102230246251565e-16, eps_lu2 = 2e-16;
-    out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; out[3] = h[3]; out[4] = h[4]; out[5] = h[5];
-    out[6] = h[0] / (eps * (h[1] * h[3] + h[5]) * n);
-    out[7] = h[0] / (eps_lu2 * h[2] * n);
-    out[8] = h[0] / (eps_lu2 * h[2] * h[4]);
-    out[9] = h[0] / (eps_lu2 * h[1] * h[3] * n);
     return LU2MI_OK;
 }

@@ -354,6 +354,31 @@ def test_lookahead_schedule_bit_identical(ctx, n, nb, mode):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n,nb,mode", [(4096, 256, L.PIV_GEPP), (2048, 512, L.PIV_COMPAT), (34816, 2048, L.PIV_GEPP)])
+def test_strict_sync_bit_identical(ctx, n, nb, mode):
+    """Publishing the leaf's exchange slots and the back substitution's
+    frontier with agent-scope release/acquire changes only cache maintenance:
+    factors, pivots and x are bit-identical to the default publication."""
+    lda = n + 16
+    outs = []
+    for strict in (0, 1):
+        dA = ctx.alloc(8 * n * lda)
+        dx = ctx.alloc(8 * n)
+        dp = ctx.alloc(4 * n)
+        ctx.randomsetmat(n, 9, dA, lda)
+        ctx.set_option(L.OPT_PIVMODE, mode)
+        ctx.set_option(L.OPT_STRICT_SYNC, strict)
+        try:
+            ctx.lumcolumn_dev(n, dA, lda, nb, dp, dx)
+        finally:
+            ctx.set_option(L.OPT_STRICT_SYNC, 0)   # the library default
+        outs.append((dA.download(np.float64, n * lda), dx.download(np.float64, n), dp.download(np.int32, n)))
+        for d in (dA, dx, dp):
+            d.free()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("n,nb,mode,sub", [
     (1024, 256, L.PIV_COMPAT, 256),
     (2048, 512, L.PIV_GEPP, 256),
