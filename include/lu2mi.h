@@ -112,6 +112,19 @@ int lu2mi_dgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, double alpha,
 int lu2mi_residual_dev(lu2mi_ctx *ctx, int n, const double *dA0, long lda,
                        const double *dx, double *out);
 
+/* lu2mi_residual_dev without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it).  The ten
+ * figures of lu2mi_residual_dev, bit for bit, go to dout (device memory, 10
+ * doubles); the work is ordered on the context's stream and the call does not
+ * wait for it.  The partial results live in a fixed array of
+ * 3 * LU2MI_RESID_NMAX doubles (3 MiB) that each device holds until the
+ * process ends, so n above LU2MI_RESID_NMAX is rejected (LU2MI_ERR_ARG);
+ * calls that overlap on one device share that array and race. */
+#define LU2MI_RESID_NMAX 131072
+int lu2mi_residual_dev_noalloc(lu2mi_ctx *ctx, int n, const double *dA0, long lda,
+                               const double *dx, double *dout);
+
 /* ---- mixed precision ---------------------------------------------------- */
 #define LU2MI_PREC_FP64 0
 #define LU2MI_PREC_FP32 1

@@ -44,6 +44,7 @@ OPT_STRICT_SYNC = 8
 BATCH_EAGER = 1
 BATCH_NMAX = 512
 BATCH_RHSMAX = 8
+RESID_NMAX = 131072
 
 _lib = None
 
@@ -100,6 +101,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "lu2mi_dgemm_dev": (c_int, [vp, c_int, c_int, c_int, c_double, vp, c_long, vp, c_long,
                                     c_double, vp, c_long]),
         "lu2mi_residual_dev": (c_int, [vp, c_int, vp, c_long, vp, vp]),
+        "lu2mi_residual_dev_noalloc": (c_int, [vp, c_int, vp, c_long, vp, vp]),
         "lu2mi_get_timing": (c_int, [vp, vp, c_int]),
         "lu2mi_solve_mixed": (c_int, [vp, c_int, vp, c_long, c_int, c_int, vp, c_int, c_double,
                                       ctypes.POINTER(IRInfo)]),
@@ -233,6 +235,11 @@ class Context:
         out = np.zeros(10)
         _check(_lib.lu2mi_residual_dev(self.handle, n, dA0.ptr, lda, dx.ptr, out.ctypes.data))
         return out
+
+    def residual_dev_noalloc(self, n, dA0, lda, dx, dout) -> None:
+        """The residual figures into dout (device, 10 doubles) without any
+        allocation; stream-ordered, not synchronized."""
+        _check(_lib.lu2mi_residual_dev_noalloc(self.handle, n, dA0.ptr, lda, dx.ptr, dout.ptr))
 
     def synchronize(self) -> None:
         _check(_lib.lu2mi_synchronize(self.handle))

@@ -66,6 +66,7 @@ extern "C" int lu2mi_init(int device, lu2mi_ctx **out)
     LU2MI_CHECK(hipMalloc(&ctx->d_err, 256));
     LU2MI_CHECK(hipMemset(ctx->d_err, 0, 256));
     ctx->bar_count = 0;
+    LU2MI_TRY(lu2mi_scratch_address(&ctx->d_scratch));
     *out = ctx;
     return LU2MI_OK;
 }
@@ -252,6 +253,13 @@ extern "C" int lu2mi_residual_dev(lu2mi_ctx *ctx, int n, const double *dA0, long
 {
     CTX_OK(ctx);
     return lu2mi_residual_impl(ctx, n, dA0, lda, dx, out);
+}
+
+extern "C" int lu2mi_residual_dev_noalloc(lu2mi_ctx *ctx, int n, const double *dA0, long lda, const double *dx,
+                                          double *dout)
+{
+    CTX_OK(ctx);
+    return lu2mi_residual_noalloc_impl(ctx, n, dA0, lda, dx, dout);
 }
 
 extern "C" int lu2mi_get_timing(lu2mi_ctx *ctx, double *out, int nout)

@@ -102,6 +102,8 @@ struct lu2mi_ctx {
     void *h_dev_a; size_t h_dev_a_bytes;
     void *h_dev_b; size_t h_dev_b_bytes;
     void *h_dev_c; size_t h_dev_c_bytes;
+    // this device's copy of the residual scratch array (lu2mi_residual_dev_noalloc)
+    double *d_scratch;
 };
 
 int lu2mi_ctx_reserve(lu2mi_ctx *ctx, int n, int nb, int elem_bytes);
@@ -123,6 +125,8 @@ int lu2mi_backsub_f32_f64(lu2mi_ctx *ctx, const float *A, long lda, int n, doubl
 int lu2mi_cm_panel_f64(lu2mi_ctx *ctx, double *dcm, long ld, int n, int m, int i, int pivmode, int *hpv);
 int lu2mi_backsub_f64_f64(lu2mi_ctx *ctx, const double *A, long lda, int n, double *x);
 int lu2mi_dev_buf(void **p, size_t *cur, size_t need);
+int lu2mi_scratch_address(double **p);
+int lu2mi_residual_noalloc_impl(lu2mi_ctx *ctx, int n, const double *dA0, long lda, const double *dx, double *dout);
 
 // default context used by the lu2-shaped (reference-signature) entry points
 lu2mi_ctx *lu2mi_default_ctx(void);

@@ -1017,6 +1017,13 @@ __global__ __launch_bounds__(256) void resid_final_kernel(const double *A, long 
     }
 }
 
+// Scratch of the residual entry point that allocates nothing
+// (lu2mi_residual_dev_noalloc): per-row |(Ax)_i - b_i| and sum_j |a_ij|, per
+// column sum_i |a_ij| -- 3n doubles for n up to LU2MI_RESID_NMAX.  Each
+// device holds its own copy until the process ends; host code reaches it only
+// through the address lu2mi_scratch_address() obtains (ctx->d_scratch), and
+// calls that overlap on one device share it (they race).
+static __device__ __align__(256) double scratch[3 * (size_t)LU2MI_RESID_NMAX];
 
 // ===========================================================================
 // host side
@@ -1627,4 +1634,30 @@ int lu2mi_residual_impl(lu2mi_ctx *ctx, int n, const double *dA0, long lda, cons
         return LU2MI_ERR_HIP;
     }
     return LU2MI_OK;
+}
+
+// address of this device's copy of scratch (called once per device, with the
+// device current, when a context is created)
+int lu2mi_scratch_address(double **p)
+{
+    *p = nullptr;
+    LU2MI_CHECK(hipGetSymbolAddress((void **)p, HIP_SYMBOL(scratch)));
+    return LU2MI_OK;
+}
+
+int lu2mi_residual_noalloc_impl(lu2mi_ctx *ctx, int n, const double *dA0, long lda, const double *dx, double *dout)
+{
+    if (!ctx || !dA0 || !dx || !dout || n <= 0 || lda < n + 1) {
+        lu2mi_set_error_msg("residual (no allocation): bad arguments");
+        return LU2MI_ERR_ARG;
+    }
+    if (n > LU2MI_RESID_NMAX) {
+        lu2mi_set_error_msg("residual (no allocation): n = %d exceeds the scratch limit %d", n, LU2MI_RESID_NMAX);
+        return LU2MI_ERR_ARG;
+    }
+    if (!ctx->d_scratch) {
+        lu2mi_set_error_msg("residual (no allocation): scratch address unknown");
+        return LU2MI_ERR_HIP;
+    }
+    return residual_launch(ctx, n, dA0, lda, dx, ctx->d_scratch, dout);
 }
