@@ -226,7 +226,7 @@ def main() -> int:
     ap.add_argument("--gemm-probe", type=int, nargs=3, metavar=("M", "N", "K"), default=None,
                     help="time the trailing-update DGEMM alone at this shape (counter runs)")
     ap.add_argument("--probe-reps", type=int, default=1)
-    ap.add_argument("--trail-kernel", type=int, default=0, choices=(0, 1),
+    ap.add_argument("--trail-kernel", type=int, default=1, choices=(0, 1),
                     help="trailing-update DGEMM tile: 1 direct-to-LDS (default), 0 register-staged")
     args = ap.parse_args()
 

@@ -52,7 +52,7 @@ extern "C" {
                                   barriers (default 0 = one launch per leaf and per GEMM,
                                   which measured faster; results identical either way) */
 #define LU2MI_OPT_TRAIL_KERNEL 6  /* trailing-update DGEMM tile: 1 = operands loaded straight
-                                     into LDS, 0 = staged through registers (default); the
+                                     into LDS (default), 0 = staged through registers; the
                                      two sum each product's K terms in different orders */
 #define LU2MI_OPT_DGEMM_TRAILING 7 /* 1: lu2mi_dgemm_dev runs its product with the
                                      trailing-update kernel (to measure that kernel alone) */

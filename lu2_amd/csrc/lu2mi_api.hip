@@ -58,7 +58,7 @@ extern "C" int lu2mi_init(int device, lu2mi_ctx **out)
     LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_next, hipEventDisableTiming));
     LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_panel, hipEventDisableTiming));
     ctx->lookahead = 1;
-    ctx->trail_kernel = 0;
+    ctx->trail_kernel = 1;
     ctx->dgemm_as_trailing = 0;
     ctx->sub_w = 0;
     LU2MI_CHECK(hipMalloc(&ctx->d_bar, 256));

@@ -45,6 +45,18 @@ def device_solve(ctx, n, nb, mode, seed=1, want_residual=True):
     return x, piv, res
 
 
+def forward_bound(n, hpl2_a, hpl2_b):
+    """How far apart two backward-stable solutions of lu2's matrix may be.
+    x1 - x2 = A^-1 (r2 - r1), so ||x1 - x2|| / ||x|| <= kappa (eta1 + eta2)
+    with eta = ||r|| / (||A|| ||x||) ~ HPL-2 * n * eps the normwise backward
+    errors.  For an n x n matrix of i.i.d. entries kappa_2 is about
+    sigma_max / sigma_min ~ (2 sqrt(n) s) / (s / sqrt(n)) = 2n; the bound
+    takes 10n.  At n = 65536 with HPL-2 ~ 2e-3 this is ~2e-8 (two correct
+    FP64 factorizations of this matrix measured ~2e-9 apart)."""
+    eps = 2.0 ** -53
+    return 10.0 * n * (hpl2_a + hpl2_b) * n * eps
+
+
 def valid_pivots(piv):
     n = len(piv)
     return bool(np.all(piv >= np.arange(n)) and np.all(piv < n))
@@ -67,9 +79,14 @@ def test_lu_large_against_oracle(ctx, n, nb, mode):
     mism = np.flatnonzero(piv != g["piv"])
     if mode == "gepp":
         assert mism.size == 0, f"{mism.size} pivots differ, first at column {mism[:5]}"
-        xo = g["x"]
-        assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
         assert res[6] < 16.0
+        xo = g["x"]
+        err = np.abs(x - xo).max() / np.abs(xo).max()
+        if n <= 32768:
+            assert err <= 1e-10      # the drop-in configuration's bar (n = 32768)
+        else:
+            # the oracle's own HPL-2 residual for this case (make_oracle_large.py)
+            assert err <= forward_bound(n, res[6], 4.46e-3), err
     else:
         assert mism.size == 0 or mism[0] >= 16384, f"pivots differ from column {mism[:5]}"
         assert mism.size <= n // 1000, f"{mism.size} pivots differ"
@@ -160,7 +177,8 @@ def test_mixed_n65536(ctx, fp64_n65536, prec):
     assert info["converged"] and info["iterations"] >= 1
     assert res[6] <= 1.0
     x64 = fp64_n65536["x"]
-    assert np.abs(x - x64).max() <= 1e-9 * np.abs(x64).max()
+    err = np.abs(x - x64).max() / np.abs(x64).max()
+    assert err <= forward_bound(N3, res[6], fp64_n65536["res"][6]), err
 
 
 N4, NB4 = 98304, 2048

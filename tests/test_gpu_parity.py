@@ -147,7 +147,7 @@ def trailing_product(ctx, a, b, c, kernel):
         ctx.dgemm_dev(m, n, k, -1.0, dA, lda, dB, ldb, 1.0, dC, ldb)
     finally:
         ctx.set_option(L.OPT_DGEMM_TRAILING, 0)
-        ctx.set_option(L.OPT_TRAIL_KERNEL, 0)   # the library default
+        ctx.set_option(L.OPT_TRAIL_KERNEL, 1)   # the library default
     out = dC.download(np.float64, m * ldb).reshape(m, ldb)
     for d in (dA, dB, dC):
         d.free()
