@@ -196,6 +196,40 @@ def gemm_probe(m: int, n: int, k: int, reps: int, trail_kernel: int) -> int:
     return 0
 
 
+def host_gemm_probe(m: int, n: int, k: int) -> int:
+    """A host-memory product through the reference's mydgemm seam
+    (lu2lib.c:2079, C -= A B on malloc'd row-major matrices): operands cross
+    PCIe through the library's pinned bounce buffers in overlapped row chunks.
+    Prints the wall time and the PCIe bytes per second of the second call
+    (the first allocates the staging)."""
+    import ctypes
+
+    import numpy as np
+
+    import lu2_amd as L
+
+    lib = ctypes.CDLL(L.LIB_PATH)
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.mydgemm.argtypes = [ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
+    lib.lu2mi_host_gemm_stats.argtypes = [vp, ci]
+    rng = np.random.default_rng(0)
+    a = rng.uniform(-0.5, 0.5, (m, k))
+    b = rng.uniform(-0.5, 0.5, (k, n))
+    c = rng.uniform(-0.5, 0.5, (m, n))
+    for _ in range(2):
+        t0 = time.perf_counter()
+        lib.mydgemm(m, n, k, -1.0, a.ctypes.data, k, b.ctypes.data, n, 1.0, c.ctypes.data, n)
+        dt = time.perf_counter() - t0
+    st = np.zeros(4)
+    lib.lu2mi_host_gemm_stats(st.ctypes.data, 4)
+    print(json.dumps({"probe": "host-memory mydgemm seam", "M": m, "N": n, "K": k, "wall_ms": dt * 1e3,
+                      "bytes_up": st[0], "bytes_down": st[1], "chunks": int(st[3]),
+                      "pcie_gbs": (st[0] + st[1]) / dt / 1e9,
+                      "pcie_peak_gbs_per_direction": 64.0,
+                      "tflops": 2.0 * m * n * k / dt / 1e12}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,12 +260,16 @@ def main() -> int:
     ap.add_argument("--gemm-probe", type=int, nargs=3, metavar=("M", "N", "K"), default=None,
                     help="time the trailing-update DGEMM alone at this shape (counter runs)")
     ap.add_argument("--probe-reps", type=int, default=1)
+    ap.add_argument("--host-gemm", type=int, nargs=3, metavar=("M", "N", "K"), default=None,
+                    help="time a host-memory product through the mydgemm seam (PCIe staging)")
     ap.add_argument("--trail-kernel", type=int, default=1, choices=(0, 1),
                     help="trailing-update DGEMM tile: 1 direct-to-LDS (default), 0 register-staged")
     args = ap.parse_args()
 
     if args.gemm_probe:
         return gemm_probe(*args.gemm_probe, args.probe_reps, args.trail_kernel)
+    if args.host_gemm:
+        return host_gemm_probe(*args.host_gemm)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the LU on a single MI355X "
               "(no multi-GPU path); run with --gpus 1", file=sys.stderr)
@@ -275,9 +313,15 @@ def main() -> int:
         dt = one_step(False)
         log(f"warmup {w}: {dt:.3f} s")
     times, gemm_ms, gemm_fl, gemm_launch = [], 0.0, 0.0, 0
+    subs = {}
     for s in range(args.steps):
         dt = one_step(True)
         tm = ctx.timing()
+        for name, v in ctx.subsystem_timing().items():
+            if isinstance(v, dict):
+                acc = subs.setdefault(name, {"ms": 0.0, "flops": 0.0, "bytes": 0.0})
+                for k in acc:
+                    acc[k] += v[k]
         times.append(dt)
         gemm_ms += tm["gemm_ms"]
         gemm_fl += tm["gemm_flops"]
@@ -365,6 +409,20 @@ def main() -> int:
             "avg_launch_ms": gemm_ms / max(1, gemm_launch),
         },
     }
+    # lu2's print_timers groups (timerlib.c:47-70) from the library's HIP
+    # events, per step; with the lookahead schedule the panel's stream runs
+    # beside the main one, so its time overlaps the others
+    out["subsystems"] = {}
+    for name, acc in subs.items():
+        ms = acc["ms"] / args.steps
+        e = {"ms_per_step": ms}
+        if acc["flops"] > 0 and ms > 0:
+            e["tflops"] = acc["flops"] / args.steps / ms / 1e9
+            e["frac_of_fp64_peak"] = e["tflops"] / FP64_MFMA_PEAK_TFLOPS
+        if acc["bytes"] > 0 and ms > 0:
+            e["gbs"] = acc["bytes"] / args.steps / ms / 1e6
+            e["frac_of_hbm"] = e["gbs"] / HBM_PEAK_GBS
+        out["subsystems"][name] = e
     if mixed is not None:
         out["mixed_precision"] = mixed
     for b in (dA0, dA, dx, dp):

@@ -217,6 +217,37 @@ int lu2mi_batch_results(lu2mi_batch *plan, double *resid, int *iters, int *piv);
  * out[3]=wall ms of the whole call. */
 int lu2mi_get_timing(lu2mi_ctx *ctx, double *out, int nout);
 
+/* Per-subsystem timing of the last lu2mi_lumcolumn_dev with LU2MI_OPT_TIMING
+ * on (lu2's print_timers groups, timerlib.c:47-70): HIP events around each
+ * subsystem's launches on the stream they run on (with the lookahead schedule
+ * the panel's stream runs beside the main one, so the times overlap).
+ * out[4c..4c+3] for subsystem c = LU2MI_T_*: GPU ms, algorithmic flops,
+ * algorithmic bytes, number of launch groups; out[4*LU2MI_T_COUNT] = wall ms
+ * of the call, out[4*LU2MI_T_COUNT+1] = the device clock in Hz.
+ *   SWAP    block-level row exchanges with the 1/d row scale (lu2.c:524-556):
+ *           32 nb (n+1-i-nb) bytes per block (both rows of each exchange
+ *           read and written)
+ *   PANEL   the recursive panel (lu2.c:1088-1120, leaves lu2tlib.c:1041-1137):
+ *           R nb^2 flops per block (R = n-i rows: the panel LU and the
+ *           inverse of its unit-lower block)
+ *   U12     the TRSM of the U row block (lu2.c:485-521): nb^2 (n+1-i-nb) flops
+ *   TRAIL   the trailing update (lu2.c:312-343): 2 M N nb flops
+ *   BACKSUB back substitution (lu2.c:110-116): n^2 flops, 4 n^2 bytes of U */
+#define LU2MI_T_SWAP    0
+#define LU2MI_T_PANEL   1
+#define LU2MI_T_U12     2
+#define LU2MI_T_TRAIL   3
+#define LU2MI_T_BACKSUB 4
+#define LU2MI_T_COUNT   5
+int lu2mi_get_subsystem_timing(lu2mi_ctx *ctx, double *out, int nout);
+
+/* The last product made with host memory through the reference's seams
+ * (mygdrdgemm, gdrblas_dgemm, gdr_dgemm_, mydgemm, matmul_for_small_nk): bytes
+ * uploaded, bytes downloaded, wall ms, row chunks.  The operands pass through
+ * pinned bounce buffers of the seams' default context in chunks whose
+ * packing, transfers and products overlap. */
+int lu2mi_host_gemm_stats(double *out, int nout);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,8 @@ BATCH_EAGER = 1
 BATCH_NMAX = 512
 BATCH_RHSMAX = 8
 RESID_NMAX = 131072
+# lu2mi_get_subsystem_timing order (LU2MI_T_*)
+SUBSYSTEMS = ("swap", "panel", "u12_trsm", "trailing_dgemm", "backsub")
 
 _lib = None
 
@@ -103,6 +105,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "lu2mi_residual_dev": (c_int, [vp, c_int, vp, c_long, vp, vp]),
         "lu2mi_residual_dev_noalloc": (c_int, [vp, c_int, vp, c_long, vp, vp]),
         "lu2mi_get_timing": (c_int, [vp, vp, c_int]),
+        "lu2mi_get_subsystem_timing": (c_int, [vp, vp, c_int]),
         "lu2mi_solve_mixed": (c_int, [vp, c_int, vp, c_long, c_int, c_int, vp, c_int, c_double,
                                       ctypes.POINTER(IRInfo)]),
         "lu2mi_sgemm_dev": (c_int, [vp, c_int, c_int, c_int, ctypes.c_float, vp, c_long, vp, c_long,
@@ -235,6 +238,17 @@ class Context:
         out = np.zeros(10)
         _check(_lib.lu2mi_residual_dev(self.handle, n, dA0.ptr, lda, dx.ptr, out.ctypes.data))
         return out
+
+    def subsystem_timing(self) -> dict:
+        """Per-subsystem GPU ms, algorithmic flops and bytes of the last
+        factorization with OPT_TIMING on (lu2mi_get_subsystem_timing)."""
+        out = np.zeros(4 * len(SUBSYSTEMS) + 2)
+        _check(_lib.lu2mi_get_subsystem_timing(self.handle, out.ctypes.data, len(out)))
+        d = {name: {"ms": out[4 * c], "flops": out[4 * c + 1], "bytes": out[4 * c + 2], "groups": int(out[4 * c + 3])}
+             for c, name in enumerate(SUBSYSTEMS)}
+        d["wall_ms"] = out[-2]
+        d["clock_hz"] = out[-1]
+        return d
 
     def residual_dev_noalloc(self, n, dA0, lda, dx, dout) -> None:
         """The residual figures into dout (device, 10 doubles) without any

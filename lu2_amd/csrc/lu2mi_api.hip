@@ -18,6 +18,8 @@
 #include "lu2mi_seams.h"
 #include <stdlib.h>
 #include <string.h>
+#include <chrono>
+#include <thread>
 #include <vector>
 
 int lu2mi_randomsetmat_impl(lu2mi_ctx *ctx, int n, int seed, double *dA, long lda);
@@ -76,9 +78,24 @@ static void free_timer(KernelTimer &kt)
     for (int e = 0; e < 2 * kt.cap; e++) (void)hipEventDestroy(kt.ev[e]);
     free(kt.ev);
     free(kt.flops);
+    free(kt.bytes);
     kt.ev = nullptr;
-    kt.flops = nullptr;
+    kt.flops = kt.bytes = nullptr;
     kt.cap = kt.used = 0;
+}
+
+// ms, flops and bytes summed over one subsystem's recorded launch groups
+static int sum_timer(const KernelTimer &kt, double *ms, double *flops, double *bytes)
+{
+    *ms = *flops = *bytes = 0.0;
+    for (int e = 0; e < kt.used; e++) {
+        float m = 0.f;
+        LU2MI_CHECK(hipEventElapsedTime(&m, kt.ev[2 * e], kt.ev[2 * e + 1]));
+        *ms += m;
+        *flops += kt.flops[e];
+        *bytes += kt.bytes[e];
+    }
+    return LU2MI_OK;
 }
 
 extern "C" int lu2mi_finalize(lu2mi_ctx *ctx)
@@ -92,7 +109,20 @@ extern "C" int lu2mi_finalize(lu2mi_ctx *ctx)
                     ctx->m_vec, ctx->m_a16, ctx->m_b16, ctx->m_s16};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
-    free_timer(ctx->tgemm);
+    for (int c = 0; c < LU2MI_T_COUNT; c++) free_timer(ctx->tcat[c]);
+    if (ctx->h_pin_bytes) {
+        (void)hipStreamSynchronize(ctx->s_up);
+        (void)hipStreamSynchronize(ctx->s_down);
+        for (int i = 0; i < 4; i++) (void)hipHostFree(ctx->h_pin[i]);
+        for (int i = 0; i < 2; i++) {
+            (void)hipEventDestroy(ctx->ev_up[i]);
+            (void)hipEventDestroy(ctx->ev_done[i]);
+            (void)hipEventDestroy(ctx->ev_down[i]);
+        }
+        (void)hipEventDestroy(ctx->ev_bdone);
+        (void)hipStreamDestroy(ctx->s_up);
+        (void)hipStreamDestroy(ctx->s_down);
+    }
     (void)hipStreamSynchronize(ctx->stream2);
     (void)hipEventDestroy(ctx->ev_next);
     (void)hipEventDestroy(ctx->ev_panel);
@@ -136,14 +166,17 @@ extern "C" int lu2mi_set_option(lu2mi_ctx *ctx, int opt, int value)
         return LU2MI_OK;
     case LU2MI_OPT_TIMING:
         ctx->timing_on = value ? 1 : 0;
-        if (ctx->timing_on && ctx->tgemm.cap == 0) {
+        for (int c = 0; ctx->timing_on && c < LU2MI_T_COUNT; c++) {
+            KernelTimer &kt = ctx->tcat[c];
+            if (kt.cap) continue;
             const int cap = 4096;
-            ctx->tgemm.ev = (hipEvent_t *)calloc(2 * cap, sizeof(hipEvent_t));
-            ctx->tgemm.flops = (double *)calloc(cap, sizeof(double));
-            if (!ctx->tgemm.ev || !ctx->tgemm.flops) return LU2MI_ERR_NOMEM;
+            kt.ev = (hipEvent_t *)calloc(2 * cap, sizeof(hipEvent_t));
+            kt.flops = (double *)calloc(cap, sizeof(double));
+            kt.bytes = (double *)calloc(cap, sizeof(double));
+            if (!kt.ev || !kt.flops || !kt.bytes) return LU2MI_ERR_NOMEM;
             LU2MI_CHECK(hipSetDevice(ctx->device));
-            for (int e = 0; e < 2 * cap; e++) LU2MI_CHECK(hipEventCreate(&ctx->tgemm.ev[e]));
-            ctx->tgemm.cap = cap;
+            for (int e = 0; e < 2 * cap; e++) LU2MI_CHECK(hipEventCreate(&kt.ev[e]));
+            kt.cap = cap;
         }
         return LU2MI_OK;
     default:
@@ -266,18 +299,35 @@ extern "C" int lu2mi_get_timing(lu2mi_ctx *ctx, double *out, int nout)
 {
     CTX_OK(ctx);
     if (!out || nout < 4) return LU2MI_ERR_ARG;
-    double ms = 0.0, fl = 0.0;
-    KernelTimer &kt = ctx->tgemm;
-    for (int e = 0; e < kt.used; e++) {
-        float m = 0.f;
-        LU2MI_CHECK(hipEventElapsedTime(&m, kt.ev[2 * e], kt.ev[2 * e + 1]));
-        ms += m;
-        fl += kt.flops[e];
-    }
+    double ms = 0.0, fl = 0.0, by = 0.0;
+    LU2MI_TRY(sum_timer(ctx->tcat[LU2MI_T_TRAIL], &ms, &fl, &by));
     out[0] = ms;
-    out[1] = kt.used;
+    out[1] = ctx->tcat[LU2MI_T_TRAIL].used;
     out[2] = fl;
     out[3] = ctx->t_last_ms[3];
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_host_gemm_stats(double *out, int nout)
+{
+    if (!out || nout < 4) return LU2MI_ERR_ARG;
+    lu2mi_ctx *ctx = lu2mi_default_ctx();
+    for (int i = 0; i < 4; i++) out[i] = ctx->h_stats[i];
+    return LU2MI_OK;
+}
+
+extern "C" int lu2mi_get_subsystem_timing(lu2mi_ctx *ctx, double *out, int nout)
+{
+    CTX_OK(ctx);
+    if (!out || nout < 4 * LU2MI_T_COUNT + 2) return LU2MI_ERR_ARG;
+    for (int c = 0; c < LU2MI_T_COUNT; c++) {
+        LU2MI_TRY(sum_timer(ctx->tcat[c], &out[4 * c], &out[4 * c + 1], &out[4 * c + 2]));
+        out[4 * c + 3] = ctx->tcat[c].used;
+    }
+    int khz = 0;
+    LU2MI_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, ctx->device));
+    out[4 * LU2MI_T_COUNT] = ctx->t_last_ms[3];
+    out[4 * LU2MI_T_COUNT + 1] = 1e3 * khz;
     return LU2MI_OK;
 }
 
@@ -308,35 +358,213 @@ static void fatal(const char *where)
 }
 
 // Row-major C = alpha*A*B + beta*C with host matrices, on the GPU.
+//
+// The operands cross PCIe through pinned bounce buffers (hipHostMalloc,
+// allocated once per context: the user's memory is never registered).  B goes
+// up first, in row chunks; then C is processed in chunks of R rows, each chunk
+// passing through: host packing of its A (and C, if beta != 0) rows into an
+// upload buffer -> DMA up (s_up) -> its product (the context's stream) -> DMA
+// down (s_down) into a download buffer -> host unpacking into the caller's C.
+// Two buffers of each kind alternate, so chunk r's packing, chunk r-1's
+// product and chunk r-2's download overlap.  Packing and unpacking large
+// chunks is split over a few host threads.
+static constexpr size_t STAGE_BYTES = 32u << 20;   // per bounce buffer
+
+static void par_rows(long rows, size_t bytes, void (*fn)(void *, long, long), void *arg)
+{
+    const int nt = bytes >= (8u << 20) ? 8 : 1;
+    if (nt == 1 || rows < nt) {
+        fn(arg, 0, rows);
+        return;
+    }
+    std::thread th[8];
+    for (int t = 0; t < nt; t++) th[t] = std::thread(fn, arg, rows * t / nt, rows * (t + 1) / nt);
+    for (int t = 0; t < nt; t++) th[t].join();
+}
+
+struct RowCopy {   // rows of width w (doubles) between strided and packed storage
+    double *dst;
+    const double *src;
+    long ldd, lds, w;
+};
+static void copy_rows(void *p, long r0, long r1)
+{
+    const RowCopy &c = *(const RowCopy *)p;
+    for (long r = r0; r < r1; r++) memcpy(c.dst + r * c.ldd, c.src + r * c.lds, sizeof(double) * c.w);
+}
+static void copy_block(double *dst, long ldd, const double *src, long lds, long rows, long w)
+{
+    RowCopy c{dst, src, ldd, lds, w};
+    par_rows(rows, sizeof(double) * rows * w, copy_rows, &c);
+}
+
+static int host_stage_init(lu2mi_ctx *ctx)
+{
+    if (ctx->h_pin_bytes) return LU2MI_OK;
+    for (int i = 0; i < 4; i++) LU2MI_CHECK(hipHostMalloc(&ctx->h_pin[i], STAGE_BYTES, hipHostMallocDefault));
+    LU2MI_CHECK(hipStreamCreateWithFlags(&ctx->s_up, hipStreamNonBlocking));
+    LU2MI_CHECK(hipStreamCreateWithFlags(&ctx->s_down, hipStreamNonBlocking));
+    for (int i = 0; i < 2; i++) {
+        LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_up[i], hipEventDisableTiming));
+        LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_done[i], hipEventDisableTiming));
+        LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_down[i], hipEventDisableTiming));
+    }
+    LU2MI_CHECK(hipEventCreateWithFlags(&ctx->ev_bdone, hipEventDisableTiming));
+    ctx->h_pin_bytes = STAGE_BYTES;
+    return LU2MI_OK;
+}
+
+// Whole host <-> device copies of the seams (lumcolumn's matrix, the
+// column-major panel) through the same bounce buffers: chunk j is copied
+// into (out of) a pinned buffer on the host while chunk j-1's DMA runs.
+struct ByteCopy {
+    char *dst;
+    const char *src;
+    size_t bytes;
+};
+static void copy_bytes(void *p, long t0, long t1)
+{
+    const ByteCopy &c = *(const ByteCopy *)p;
+    const size_t a = c.bytes * (size_t)t0 / 8, b = c.bytes * (size_t)t1 / 8;
+    memcpy(c.dst + a, c.src + a, b - a);
+}
+static void par_copy(void *dst, const void *src, size_t bytes)
+{
+    ByteCopy c{(char *)dst, (const char *)src, bytes};
+    par_rows(8, bytes, copy_bytes, &c);   // eight slices (one thread each when large)
+}
+
+static int staged_h2d(lu2mi_ctx *ctx, void *d, const void *h, size_t bytes)
+{
+    LU2MI_TRY(host_stage_init(ctx));
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    int j = 0;
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES, j++) {
+        const size_t len = bytes - off < STAGE_BYTES ? bytes - off : STAGE_BYTES;
+        if (j >= 2) LU2MI_CHECK(hipEventSynchronize(ctx->ev_up[j & 1]));
+        par_copy(ctx->h_pin[j & 1], (const char *)h + off, len);
+        LU2MI_CHECK(hipMemcpyAsync((char *)d + off, ctx->h_pin[j & 1], len, hipMemcpyHostToDevice, ctx->s_up));
+        LU2MI_CHECK(hipEventRecord(ctx->ev_up[j & 1], ctx->s_up));
+    }
+    LU2MI_CHECK(hipStreamSynchronize(ctx->s_up));
+    return LU2MI_OK;
+}
+
+static int staged_d2h(lu2mi_ctx *ctx, void *h, const void *d, size_t bytes)
+{
+    LU2MI_TRY(host_stage_init(ctx));
+    LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
+    const size_t nch = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
+    for (size_t j = 0; j <= nch; j++) {
+        if (j < nch) {
+            const size_t off = j * STAGE_BYTES, len = bytes - off < STAGE_BYTES ? bytes - off : STAGE_BYTES;
+            // the download buffer was emptied when chunk j-2 was unpacked
+            LU2MI_CHECK(hipMemcpyAsync(ctx->h_pin[2 + (j & 1)], (const char *)d + off, len, hipMemcpyDeviceToHost,
+                                       ctx->s_down));
+            LU2MI_CHECK(hipEventRecord(ctx->ev_down[j & 1], ctx->s_down));
+        }
+        if (j >= 1) {
+            const size_t off = (j - 1) * STAGE_BYTES, len = bytes - off < STAGE_BYTES ? bytes - off : STAGE_BYTES;
+            LU2MI_CHECK(hipEventSynchronize(ctx->ev_down[(j - 1) & 1]));
+            par_copy((char *)h + off, ctx->h_pin[2 + ((j - 1) & 1)], len);
+        }
+    }
+    return LU2MI_OK;
+}
+
 static int host_gemm_rm(int m, int n, int k, double alpha, const double *a, long na, const double *b, long nb,
                         double beta, double *c, long nc)
 {
     if (m <= 0 || n <= 0) return LU2MI_OK;
     lu2mi_ctx *ctx = lu2mi_default_ctx();
     LU2MI_CHECK(hipSetDevice(ctx->device));
-    const size_t kk = k > 0 ? (size_t)k : 1;
-    LU2MI_TRY(lu2mi_dev_buf(&ctx->h_dev_a, &ctx->h_dev_a_bytes, sizeof(double) * (size_t)m * kk));
-    LU2MI_TRY(lu2mi_dev_buf(&ctx->h_dev_b, &ctx->h_dev_b_bytes, sizeof(double) * kk * (size_t)n));
-    LU2MI_TRY(lu2mi_dev_buf(&ctx->h_dev_c, &ctx->h_dev_c_bytes, sizeof(double) * (size_t)m * (size_t)n));
-    double *dA = (double *)ctx->h_dev_a, *dB = (double *)ctx->h_dev_b, *dC = (double *)ctx->h_dev_c;
-    hipStream_t s = ctx->stream;
-    if (k > 0) {
-        LU2MI_CHECK(hipMemcpy2DAsync(dA, sizeof(double) * k, a, sizeof(double) * na, sizeof(double) * k, m,
-                                     hipMemcpyHostToDevice, s));
-        LU2MI_CHECK(hipMemcpy2DAsync(dB, sizeof(double) * n, b, sizeof(double) * nb, sizeof(double) * n, k,
-                                     hipMemcpyHostToDevice, s));
+    LU2MI_TRY(host_stage_init(ctx));
+    const auto t0 = std::chrono::steady_clock::now();
+    const long K = k > 0 ? k : 0, kk = K > 0 ? K : 1;
+    const size_t row_up = sizeof(double) * (size_t)(K + (beta != 0.0 ? n : 0)), row_c = sizeof(double) * n;
+    // rows of C per chunk: one chunk's A and C rows fit an upload buffer, its
+    // C rows a download buffer
+    long R = (long)(STAGE_BYTES / (row_up > row_c ? row_up : row_c));
+    if (R < 1) {
+        lu2mi_set_error_msg("host gemm: a row of %ld doubles exceeds the %zu-byte bounce buffer",
+                            (long)(row_up / sizeof(double)), STAGE_BYTES);
+        return LU2MI_ERR_UNSUPPORTED;
     }
-    if (beta != 0.0)
-        LU2MI_CHECK(hipMemcpy2DAsync(dC, sizeof(double) * n, c, sizeof(double) * nc, sizeof(double) * n, m,
-                                     hipMemcpyHostToDevice, s));
-    GemmDesc g;
-    g.M = m; g.N = n; g.K = k > 0 ? k : 0; g.batch = 1; g.alpha = alpha; g.beta = beta;
-    g.A = dA; g.lda = kk; g.sA = 0; g.B = dB; g.ldb = n; g.sB = 0; g.C = dC; g.ldc = n; g.sC = 0;
-    g.a_lower = 0;
-    LU2MI_TRY(lu2mi_gemm_f64(g, s));
-    LU2MI_CHECK(hipMemcpy2DAsync(c, sizeof(double) * nc, dC, sizeof(double) * n, sizeof(double) * n, m,
-                                 hipMemcpyDeviceToHost, s));
-    LU2MI_CHECK(hipStreamSynchronize(s));
+    if (R > m) R = m;
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->h_dev_a, &ctx->h_dev_a_bytes, sizeof(double) * 2 * (size_t)R * kk));
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->h_dev_b, &ctx->h_dev_b_bytes, sizeof(double) * (size_t)kk * n));
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->h_dev_c, &ctx->h_dev_c_bytes, sizeof(double) * 2 * (size_t)R * n));
+    double *dB = (double *)ctx->h_dev_b;
+    double *dA[2] = {(double *)ctx->h_dev_a, (double *)ctx->h_dev_a + (size_t)R * kk};
+    double *dC[2] = {(double *)ctx->h_dev_c, (double *)ctx->h_dev_c + (size_t)R * n};
+    double *pin[2] = {(double *)ctx->h_pin[0], (double *)ctx->h_pin[1]};
+    double *pout[2] = {(double *)ctx->h_pin[2], (double *)ctx->h_pin[3]};
+    hipStream_t su = ctx->s_up, sc = ctx->stream, sd = ctx->s_down;
+    double up = 0.0, down = 0.0;
+    // previous users of the bounce buffers and chunk slots are done
+    LU2MI_CHECK(hipStreamSynchronize(sc));
+    // B, in row chunks through the two upload buffers
+    if (K > 0) {
+        const long Rb = (long)(STAGE_BYTES / (sizeof(double) * n)) > 0 ? (long)(STAGE_BYTES / (sizeof(double) * n)) : 1;
+        int j = 0;
+        for (long r0 = 0; r0 < K; r0 += Rb, j++) {
+            const long rows = (K - r0 < Rb) ? K - r0 : Rb;
+            if (j >= 2) LU2MI_CHECK(hipEventSynchronize(ctx->ev_up[j & 1]));
+            copy_block(pin[j & 1], n, b + r0 * nb, nb, rows, n);
+            LU2MI_CHECK(hipMemcpyAsync(dB + r0 * n, pin[j & 1], sizeof(double) * rows * n, hipMemcpyHostToDevice,
+                                       su));
+            LU2MI_CHECK(hipEventRecord(ctx->ev_up[j & 1], su));
+            up += sizeof(double) * (double)rows * n;
+        }
+        LU2MI_CHECK(hipEventRecord(ctx->ev_bdone, su));
+        LU2MI_CHECK(hipStreamWaitEvent(sc, ctx->ev_bdone, 0));
+        // the upload buffers are free once B's last chunks have gone up
+        LU2MI_CHECK(hipEventSynchronize(ctx->ev_bdone));
+    }
+    const long nchunks = (m + R - 1) / R;
+    for (long q = 0; q <= nchunks; q++) {
+        if (q < nchunks) {
+            const int s = (int)(q & 1);
+            const long r0 = q * R, rows = (m - r0 < R) ? m - r0 : R;
+            // slot s was last used by chunk q-2: its download (hence its
+            // product and upload) is complete
+            if (q >= 2) LU2MI_CHECK(hipEventSynchronize(ctx->ev_down[s]));
+            double *pa = pin[s], *pc = pin[s] + (size_t)rows * kk;
+            if (K > 0) copy_block(pa, K, a + r0 * na, na, rows, K);
+            if (beta != 0.0) copy_block(pc, n, c + r0 * nc, nc, rows, n);
+            if (K > 0) {
+                LU2MI_CHECK(hipMemcpyAsync(dA[s], pa, sizeof(double) * rows * K, hipMemcpyHostToDevice, su));
+                up += sizeof(double) * (double)rows * K;
+            }
+            if (beta != 0.0) {
+                LU2MI_CHECK(hipMemcpyAsync(dC[s], pc, sizeof(double) * rows * n, hipMemcpyHostToDevice, su));
+                up += sizeof(double) * (double)rows * n;
+            }
+            LU2MI_CHECK(hipEventRecord(ctx->ev_up[s], su));
+            LU2MI_CHECK(hipStreamWaitEvent(sc, ctx->ev_up[s], 0));
+            GemmDesc g;
+            g.M = (int)rows; g.N = n; g.K = (int)K; g.batch = 1; g.alpha = alpha; g.beta = beta;
+            g.A = dA[s]; g.lda = kk; g.sA = 0; g.B = dB; g.ldb = n; g.sB = 0; g.C = dC[s]; g.ldc = n; g.sC = 0;
+            g.a_lower = 0;
+            LU2MI_TRY(lu2mi_gemm_f64(g, sc));
+            LU2MI_CHECK(hipEventRecord(ctx->ev_done[s], sc));
+            LU2MI_CHECK(hipStreamWaitEvent(sd, ctx->ev_done[s], 0));
+            LU2MI_CHECK(hipMemcpyAsync(pout[s], dC[s], sizeof(double) * rows * n, hipMemcpyDeviceToHost, sd));
+            LU2MI_CHECK(hipEventRecord(ctx->ev_down[s], sd));
+            down += sizeof(double) * (double)rows * n;
+        }
+        if (q >= 1) {   // chunk q-1 back into the caller's C
+            const int s = (int)((q - 1) & 1);
+            const long r0 = (q - 1) * R, rows = (m - r0 < R) ? m - r0 : R;
+            LU2MI_CHECK(hipEventSynchronize(ctx->ev_down[s]));
+            copy_block(c + r0 * nc, nc, pout[s], n, rows, n);
+        }
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    ctx->h_stats[0] = up;
+    ctx->h_stats[1] = down;
+    ctx->h_stats[2] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ctx->h_stats[3] = (double)nchunks;
     return LU2MI_OK;
 }
 
@@ -483,9 +711,9 @@ extern "C" void cm_column_decomposition_recursive(int n, double *a, int m, int *
     const size_t bytes = sizeof(double) * (size_t)m * (size_t)n;
     double *d = nullptr;
     if (lu2mi_malloc(ctx, (void **)&d, bytes)) fatal("cm_column_decomposition_recursive allocation");
-    if (lu2mi_memcpy_h2d(ctx, d, a, bytes)) fatal("cm_column_decomposition_recursive copy-in");
+    if (staged_h2d(ctx, d, a, bytes)) fatal("cm_column_decomposition_recursive copy-in");
     if (lu2mi_cm_panel_f64(ctx, d, n, n, m, i, ctx->pivmode, pv)) fatal("cm_column_decomposition_recursive");
-    if (lu2mi_memcpy_d2h(ctx, a, d, bytes)) fatal("cm_column_decomposition_recursive copy-out");
+    if (staged_d2h(ctx, a, d, bytes)) fatal("cm_column_decomposition_recursive copy-out");
     lu2mi_free(ctx, d);
 }
 
@@ -509,10 +737,10 @@ extern "C" void lumcolumn(int n, double *a, double *b, int m, double *awork, int
     if (lu2mi_malloc(ctx, (void **)&dA, bytes) || lu2mi_malloc(ctx, (void **)&dx, sizeof(double) * n) ||
         lu2mi_malloc(ctx, (void **)&dp, sizeof(int) * n))
         fatal("lumcolumn allocation");
-    if (lu2mi_memcpy_h2d(ctx, dA, a, bytes)) fatal("lumcolumn copy-in");
+    if (staged_h2d(ctx, dA, a, bytes)) fatal("lumcolumn copy-in");
     if (lu2mi_factor_f64(ctx, n, dA, lda, m, ctx->pivmode, dp, dx)) fatal("lumcolumn factorization");
     std::vector<int> piv(n);
-    if (lu2mi_memcpy_d2h(ctx, a, dA, bytes) || lu2mi_memcpy_d2h(ctx, b, dx, sizeof(double) * n) ||
+    if (staged_d2h(ctx, a, dA, bytes) || lu2mi_memcpy_d2h(ctx, b, dx, sizeof(double) * n) ||
         lu2mi_memcpy_d2h(ctx, piv.data(), dp, sizeof(int) * n))
         fatal("lumcolumn copy-out");
     for (int k = 0; k < m && k < n; k++) pv[k] = piv[n - m + k];

@@ -46,9 +46,10 @@ int lu2mi_gemm_f32(const GemmDesc &g, hipStream_t s);
 // ---------------------------------------------------------------------------
 // Per-context device workspace for the GPU-resident factorization.
 struct KernelTimer {
-    // events around every trailing-update GEMM launch, read after sync
+    // events around each launch group of one subsystem, read after sync
     hipEvent_t *ev;      // 2 * cap
-    double *flops;       // cap
+    double *flops;       // cap: algorithmic flops of each group
+    double *bytes;       // cap: algorithmic bytes of each group
     int cap, used;
 };
 
@@ -85,7 +86,7 @@ struct lu2mi_ctx {
     int strict_sync;       // leaf and back substitution publish with release/acquire
     hipStream_t stream2;
     hipEvent_t ev_next, ev_panel;
-    KernelTimer tgemm;
+    KernelTimer tcat[LU2MI_T_COUNT];   // per subsystem (LU2MI_T_*)
     double t_last_ms[8];
     // mixed precision: FP32 working copy, kept per-block factors, refinement vectors
     void *m_af; size_t m_af_bytes;
@@ -98,10 +99,18 @@ struct lu2mi_ctx {
     void *m_a16; size_t m_a16_bytes;
     void *m_b16; size_t m_b16_bytes;
     void *m_s16; size_t m_s16_bytes;
-    // host staging buffers for the host-memory seams (mygdrdgemm, lumcolumn)
+    // host-memory seams (mygdrdgemm, mydgemm, gdr_dgemm_, ...): device
+    // buffers (two row chunks of A and C, all of B), pinned bounce buffers
+    // (two for uploads, two for downloads), the two copy streams and the
+    // events that chain chunk uploads, products and downloads
     void *h_dev_a; size_t h_dev_a_bytes;
     void *h_dev_b; size_t h_dev_b_bytes;
     void *h_dev_c; size_t h_dev_c_bytes;
+    void *h_pin[4];        // [0,1] uploads, [2,3] downloads; h_pin_bytes each
+    size_t h_pin_bytes;
+    hipStream_t s_up, s_down;
+    hipEvent_t ev_up[2], ev_done[2], ev_down[2], ev_bdone;
+    double h_stats[4];     // last host-memory product: bytes up, bytes down, ms, chunks
     // this device's copy of the residual scratch array (lu2mi_residual_dev_noalloc)
     double *d_scratch;
 };

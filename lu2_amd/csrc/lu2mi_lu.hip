@@ -1354,21 +1354,62 @@ struct Factor {
         }
     }
 
+    // HIP events around one subsystem's launches on the current stream
+    // (recorded only with timing on; slot -1 = not recorded)
+    int tbegin(int cat, int &slot)
+    {
+        slot = -1;
+        KernelTimer &kt = ctx->tcat[cat];
+        if (!ctx->timing_on || kt.used >= kt.cap) return LU2MI_OK;
+        slot = kt.used++;
+        LU2MI_CHECK(hipEventRecord(kt.ev[2 * slot], ctx->stream));
+        return LU2MI_OK;
+    }
+    int tend(int cat, int slot, double flops, double bytes)
+    {
+        if (slot < 0) return LU2MI_OK;
+        KernelTimer &kt = ctx->tcat[cat];
+        LU2MI_CHECK(hipEventRecord(kt.ev[2 * slot + 1], ctx->stream));
+        kt.flops[slot] = flops;
+        kt.bytes[slot] = bytes;
+        return LU2MI_OK;
+    }
+
     int start_panel(int i)
     {
         set_block(i);
+        int ts;
+        LU2MI_TRY(tbegin(LU2MI_T_PANEL, ts));
         hipLaunchKernelGGL(fill_kernel<T>, dim3(256), dim3(256), 0, ctx->stream, linv, (long)nb * nb, T(0));
         LU2MI_CHECK(hipGetLastError());
-        return panel(i, i, nb);
+        LU2MI_TRY(panel(i, i, nb));
+        return tend(LU2MI_T_PANEL, ts, (double)(n - i) * nb * nb, 0.0);
+    }
+
+    // block-level row permutation of columns [c0, c1) with its timing
+    int block_perm(int i, int c0, int c1, T *wb, long ldw)
+    {
+        int ts;
+        LU2MI_TRY(tbegin(LU2MI_T_SWAP, ts));
+        LU2MI_TRY(rowperm(pairs, pairs + 2 * nb, 2 * nb, c0, c1, 0, 0, n + 1, i, i + nb, wb, ldw, i, i + nb));
+        return tend(LU2MI_T_SWAP, ts, 0.0, 32.0 * nb * (double)(c1 - c0));
+    }
+
+    // U12 of columns [c0, c0+N) = inv(D^-1 L11) * staged rows (lu2.c:511-512)
+    int u12(int i, int c0, int N, const T *wb, long ldw)
+    {
+        int ts;
+        LU2MI_TRY(tbegin(LU2MI_T_U12, ts));
+        LU2MI_TRY(gemm_call<T>(ctx, nb, N, nb, 1.0, linv, nb, wb, ldw, 0.0, A + (long)i * lda + c0, lda, 1));
+        return tend(LU2MI_T_U12, ts, (double)nb * nb * N, 0.0);
     }
 
     // A22[:, c0 .. c0+N) -= L21 * U12[:, c0 .. c0+N) for block i (lu2.c:580), timed
     int trailing(int i, int c0, int N)
     {
         const int M = n - i - nb;
-        KernelTimer &kt = ctx->tgemm;
-        const bool tm = ctx->timing_on && kt.used < kt.cap;
-        if (tm) LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used], ctx->stream));
+        int ts;
+        LU2MI_TRY(tbegin(LU2MI_T_TRAIL, ts));
         if (sizeof(T) == 4 && ctx->lowp)   // operands rounded to FP16/BF16, FP32 accumulation
             LU2MI_TRY(lu2mi_update16(ctx, ctx->lowp, M, N, nb, (const float *)(A + (long)(i + nb) * lda + i), lda,
                                      (const float *)(A + (long)i * lda + c0), lda,
@@ -1376,12 +1417,7 @@ struct Factor {
         else
             LU2MI_TRY(gemm_call<T>(ctx, M, N, nb, -1.0, A + (long)(i + nb) * lda + i, lda, A + (long)i * lda + c0,
                                    lda, 1.0, A + (long)(i + nb) * lda + c0, lda, 0, ctx->trail_kernel ? 1 : 2));
-        if (tm) {
-            LU2MI_CHECK(hipEventRecord(kt.ev[2 * kt.used + 1], ctx->stream));
-            kt.flops[kt.used] = 2.0 * M * (double)N * nb;
-            kt.used++;
-        }
-        return LU2MI_OK;
+        return tend(LU2MI_T_TRAIL, ts, 2.0 * M * (double)N * nb, 0.0);
     }
 
     // Block loop of lu2.c:1122-1144.  With lookahead, block i's row
@@ -1403,7 +1439,6 @@ struct Factor {
                 panel_pending = false;
             }
             set_block(i);
-            T *const linv_i = linv;
             int *const pairs_i = pairs;
             const int ncols = n + 1 - (i + nb);
             const int M = n - i - nb;
@@ -1414,11 +1449,9 @@ struct Factor {
                                ctx->d_orig, i, nb, pairs_i, pairs_i + 2 * nb);
             LU2MI_CHECK(hipGetLastError());
             const long ldw = (ncols + 3) & ~3L;   // 16-byte rows for FP64 and FP32
-            LU2MI_TRY(rowperm(pairs_i, pairs_i + 2 * nb, 2 * nb, i + nb, i + nb + na, 0, 0, n + 1, i, i + nb, wbuf,
-                              ldw, i, i + nb));
+            LU2MI_TRY(block_perm(i, i + nb, i + nb + na, wbuf, ldw));
             // U12 = inv(D^-1 L11) * staged rows   (lu2.c:511-512)
-            LU2MI_TRY(gemm_call<T>(ctx, nb, na, nb, 1.0, linv_i, nb, wbuf, ldw, 0.0, A + (long)i * lda + i + nb,
-                                   lda, 1));
+            LU2MI_TRY(u12(i, i + nb, na, wbuf, ldw));
             if (M <= 0) continue;
             if (la) {
                 LU2MI_TRY(trailing(i, i + nb, nb));                    // the next panel's columns
@@ -1432,10 +1465,9 @@ struct Factor {
                 panel_pending = true;
                 // the rest of block i, b included: staging rows from column nb
                 // of w on, so that w keeps the layout of the unsplit product
-                LU2MI_TRY(rowperm(pairs_i, pairs_i + 2 * nb, 2 * nb, i + 2 * nb, n + 1, 0, 0, n + 1, i, i + nb,
-                                  wbuf + nb, ldw, i, i + nb));
-                LU2MI_TRY(gemm_call<T>(ctx, nb, ncols - nb, nb, 1.0, linv_i, nb, wbuf + nb, ldw, 0.0,
-                                       A + (long)i * lda + i + 2 * nb, lda, 1));
+                set_block(i);
+                LU2MI_TRY(block_perm(i, i + 2 * nb, n + 1, wbuf + nb, ldw));
+                LU2MI_TRY(u12(i, i + 2 * nb, ncols - nb, wbuf + nb, ldw));
                 LU2MI_TRY(trailing(i, i + 2 * nb, ncols - nb));
             } else {
                 LU2MI_TRY(trailing(i, i + nb, ncols));
@@ -1444,9 +1476,12 @@ struct Factor {
         }
         if (panel_pending) LU2MI_CHECK(hipStreamWaitEvent(s1, ctx->ev_panel, 0));
         // x = column n, then unit-upper back substitution
+        int ts;
+        LU2MI_TRY(tbegin(LU2MI_T_BACKSUB, ts));
         hipLaunchKernelGGL(copy_col_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A, lda, n, n, dx);
         LU2MI_CHECK(hipGetLastError());
         LU2MI_TRY(backsub(ctx, A, lda, n, dx));
+        LU2MI_TRY(tend(LU2MI_T_BACKSUB, ts, (double)n * n, 4.0 * (double)n * n));
         if (dpiv_out && dpiv_out != ctx->d_piv)
             LU2MI_CHECK(hipMemcpyAsync(dpiv_out, ctx->d_piv, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
         return LU2MI_OK;
@@ -1476,7 +1511,7 @@ static int factor_impl(lu2mi_ctx *ctx, int n, T *dA, long lda, int nb, int pivmo
     f.linv_base = (T *)ctx->d_linv; f.pairs_base = ctx->d_pairs;
     f.linv_all = keep_linv;
     f.pairs_all = keep_pairs;
-    ctx->tgemm.used = 0;
+    for (int c = 0; c < LU2MI_T_COUNT; c++) ctx->tcat[c].used = 0;
     LU2MI_CHECK(hipMemsetAsync(ctx->d_err, 0, sizeof(int), ctx->stream));
     // slot sequence numbers restart with every factorization
     LU2MI_CHECK(hipMemsetAsync(ctx->d_slots, 0, ctx->slots_bytes, ctx->stream));

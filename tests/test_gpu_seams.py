@@ -104,6 +104,28 @@ def test_gdrblas_dgemm(lib, order, ta, tb):
     assert np.allclose(Cs, want, rtol=1e-13, atol=1e-12)
 
 
+@pytest.mark.parametrize("m,n,k,beta", [(5000, 2000, 600, 1.0), (3001, 777, 64, 0.0), (40, 30, 0, 0.5)])
+def test_host_gemm_chunked(lib, m, n, k, beta):
+    """Host-memory products pass through pinned bounce buffers in row chunks
+    (32 MiB each; 5000 x (600 + 2000) doubles is four chunks) whose packing,
+    transfers and products overlap: the result is the product, every chunk
+    lands in its rows, and the bytes moved are those of the operands."""
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((m, k + 3))
+    B = rng.standard_normal((k, n + 5))
+    C = rng.standard_normal((m, n + 7))
+    want = -1.0 * A[:, :k] @ B[:, :n] + beta * C[:, :n]
+    lib.mydgemm(m, n, k, -1.0, ptr(A), k + 3, ptr(B), n + 5, beta, ptr(C), n + 7)
+    assert np.allclose(C[:, :n], want, rtol=1e-12, atol=1e-11)
+    st = np.zeros(4)
+    lib.lu2mi_host_gemm_stats.argtypes = [ctypes.c_void_p, c_int]
+    assert lib.lu2mi_host_gemm_stats(st.ctypes.data, 4) == 0
+    up = 8.0 * (k * n + m * k + (m * n if beta != 0 else 0))
+    assert st[0] == up and st[1] == 8.0 * m * n
+    rows = (32 << 20) // (8 * max(k + (n if beta != 0 else 0), n))
+    assert st[3] == -(-m // min(rows, m))
+
+
 def test_matmul_for_small_nk(lib):
     rng = np.random.default_rng(2)
     m, kk, n = 300, 16, 8
@@ -167,15 +189,35 @@ def test_cm_column_decomposition_recursive(lib, oracle, n, m, i):
     assert np.array_equal(cm[:, :i], cm_in[:, :i])     # rows above i untouched
 
 
-def test_driver_lu2mi(oracle):
-    """The lu2-style driver: CLI, Emax/Nswap/Gflops lines, HPL residual."""
+@pytest.mark.parametrize("pivot", ["gepp", "compat"])
+def test_driver_lu2mi(oracle, tmp_path, pivot):
+    """The lu2-style driver: CLI, the Emax / Nswap...Gflops / HPL lines and
+    lu2's per-subsystem timer report.  Its Emax (lu2.c:45-60, evaluated on
+    the device against the regenerated matrix) is checked against the
+    oracle's Emax of the same x: two evaluations of max|Ax - b| differ by at
+    most 2 gamma_n (|A||x| + |b|), i.e. 2 units of the HPL-2 scale."""
+    import re
+
+    n, nb = 2048, 256
     exe = os.path.join(ROOT, "lu2_amd", "lu2mi")
-    r = subprocess.run([exe, "-n", "2048", "-b", "256", "-s", "1", "-p", "gepp"], capture_output=True,
-                       text=True, timeout=120)
+    xfile = tmp_path / "x.bin"
+    r = subprocess.run([exe, "-n", str(n), "-b", str(nb), "-s", "1", "-p", pivot, "-x", str(xfile)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     out = r.stdout
-    assert "N=2048 Seed=1 NB=256" in out
-    assert "Emax=" in out and "Gflops" in out and "PASSED" in out
+    assert f"N={n} Seed=1 NB={nb}" in out
+    assert re.search(r"Nswap=\d+ cpsec =\s+\S+ wsec=\S+ \S+ Gflops", out), out
+    if pivot == "gepp":
+        assert "PASSED" in out
+    for name in ("swaprows", "col dec t", "solve tri", "DGEMMrest", "back sub", "Total"):
+        m = re.search(name + r"\s+time=\s*(\S+) ops/cycle=(\S+)", out)
+        assert m and float(m.group(1)) > 0 and float(m.group(2)) > 0, (name, out)
+    emax = float(re.search(r"Emax=\s*(\S+)", out).group(1))
+    x = np.fromfile(xfile, dtype=np.float64)
+    a0 = oracle.randomsetmat(n, 1)
+    res = oracle.residuals(a0, x)
+    unit = 2.0 ** -53 * (res[1] * res[3] + res[5]) * n
+    assert abs(emax - oracle.emax(a0, x)) <= 2 * unit + 1e-3 * emax, (emax, oracle.emax(a0, x), unit)
 
 
 DROPIN = os.path.join(ROOT, "oracle", "_ref", "lu2_dropin")
