@@ -34,7 +34,8 @@ per-system drand48 streams, in one graph-captured persistent launch with the
 autotuned shape; FP64 and FP32-factor + FP64-refinement forms.
 
 cpu_baseline: the CPU oracle (a C restatement of lu2's algorithm, OpenMP) on
-a bounded sample, n=16384, on the host cores.
+the host cores: lu2's own n=2048 nb=256 and a bounded sample, n=16384 (the
+value); --cpu-full adds n=32768 nb=512.
 """
 from __future__ import annotations
 
@@ -57,21 +58,45 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, nb: int, pivot: int):
-    import numpy as np
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(n: int, nb: int, pivot: int, full: bool):
+    """The CPU oracle (lu2o_lumcolumn: lu2's blocked recursive LU, OpenMP,
+    AVX2) on the host cores: lu2's own configuration n=2048 nb=256, then the
+    bounded sample n x nb that gives `value` (and, with full, the drop-in
+    configuration's size n=32768 nb=512, about 1.5-3 minutes).  Timed region
+    as lu2.c:1287-1296: factorization and back substitution."""
     from oracle import lu2_oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    a0 = O.randomsetmat(n, 1)
-    t = time.perf_counter()
-    x, _, _ = O.lumcolumn(a0, nb, pivot)
-    dt = time.perf_counter() - t
-    res = O.residuals(a0, x)
-    flops = 2.0 * n ** 3 / 3.0
-    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"oracle lumcolumn (C restatement of lu2.c:1122, OpenMP) n={n} nb={nb} "
-                      f"seed 1, {dt:.2f} s wall, HPL-2 residual {res[6]:.3g}"}
+    runs = []
+    sizes = [(2048, 256), (n, nb)] + ([(32768, 512)] if full else [])
+    for nn, bb in sizes:
+        a0 = O.randomsetmat(nn, 1)
+        t = time.perf_counter()
+        x, _, _ = O.lumcolumn(a0, bb, pivot)
+        dt = time.perf_counter() - t
+        res = O.residuals(a0, x)
+        runs.append({"n": nn, "nb": bb, "s": dt, "tflops": 2.0 * nn ** 3 / 3.0 / dt / 1e12,
+                     "hpl2_residual": float(res[6])})
+        log(f"cpu oracle n={nn} nb={bb}: {dt:.2f} s {runs[-1]['tflops'] * 1e3:.1f} GFLOP/s")
+        del a0, x
+    main = runs[1]
+    return {"value": main["tflops"], "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"oracle lumcolumn (C restatement of lu2.c:1122, OpenMP, {threads} threads on "
+                      f"{cpu_model()}) n={main['n']} nb={main['nb']} seed 1, {main['s']:.2f} s wall, "
+                      f"HPL-2 residual {main['hpl2_residual']:.3g}",
+            "runs": runs}
 
 
 def lowp_config(ctx, n: int, nb: int) -> dict:
@@ -241,6 +266,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=16384)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time the CPU oracle at n=32768 nb=512 (minutes)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=1)
     ap.add_argument("--mixed-nb", type=int, default=2048)
@@ -255,6 +282,8 @@ def main() -> int:
     ap.add_argument("--subpanel", type=int, default=None,
                     help="widest panel part factored by one persistent launch (0 = one launch per "
                          "step; library default if omitted)")
+    ap.add_argument("--wide16", type=int, default=0, choices=(0, 1),
+                    help="16-bit trailing updates in 256 x 256 tiles (1) or 128 x 128 (0, default)")
     ap.add_argument("--strict-sync", type=int, default=0, choices=(0, 1),
                     help="leaf and back substitution publish with agent-scope release/acquire")
     ap.add_argument("--gemm-probe", type=int, nargs=3, metavar=("M", "N", "K"), default=None,
@@ -290,6 +319,7 @@ def main() -> int:
     ctx.set_option(L.OPT_LEAF_W, args.leaf_w)
     ctx.set_option(L.OPT_TRAIL_KERNEL, args.trail_kernel)
     ctx.set_option(L.OPT_STRICT_SYNC, args.strict_sync)
+    ctx.set_option(L.OPT_WIDE16, args.wide16)
     if args.subpanel is not None:
         ctx.set_option(L.OPT_SUBPANEL, args.subpanel)
     nbytes = 8 * n * lda
@@ -387,7 +417,7 @@ def main() -> int:
                                f"pivot={args.pivot}, 1 RHS, seed {args.seed}",
                    "n": n, "nb": nb, "pivot": args.pivot, "lookahead": args.lookahead, "leaf_w": args.leaf_w,
                    "subpanel": args.subpanel if args.subpanel is not None else 0,
-                   "trail_kernel": args.trail_kernel, "strict_sync": args.strict_sync},
+                   "trail_kernel": args.trail_kernel, "strict_sync": args.strict_sync, "wide16": args.wide16},
         "tflops_2n3_3_plus_2n2": (flops + 2.0 * n * n) / mean / 1e12,
         "frac_of_fp64_peak": flops / mean / 1e12 / FP64_MFMA_PEAK_TFLOPS,
         "hpl2_residual": float(res[6]),
@@ -434,7 +464,7 @@ def main() -> int:
     ctx.close()
     if not args.no_cpu:
         log(f"cpu baseline: oracle n={args.cpu_n}")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n, 256, L.PIV_GEPP)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, 256, L.PIV_GEPP, args.cpu_full)
     print(json.dumps(out), flush=True)
     return 0 if res[6] < 16 else 1
 

@@ -61,6 +61,9 @@ extern "C" {
                                      (an L2 writeback and invalidate per publication); default
                                      0: device-coherent stores completed in order (results
                                      identical either way) */
+#define LU2MI_OPT_WIDE16 9        /* 1: 16-bit trailing updates in 256 x 256 tiles (128 x 64 per
+                                     wave) where they give every CU a workgroup; 0 (default):
+                                     128 x 128 tiles (results identical either way) */
 
 typedef struct lu2mi_ctx lu2mi_ctx;
 

@@ -41,6 +41,7 @@ OPT_SUBPANEL = 5
 OPT_TRAIL_KERNEL = 6
 OPT_DGEMM_TRAILING = 7
 OPT_STRICT_SYNC = 8
+OPT_WIDE16 = 9
 BATCH_EAGER = 1
 BATCH_NMAX = 512
 BATCH_RHSMAX = 8

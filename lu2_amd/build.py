@@ -13,7 +13,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["lu2mi_gemm.hip", "lu2mi_lu.hip", "lu2mi_mixed.hip", "lu2mi_batched.hip", "lu2mi_lowp.hip", "lu2mi_api.hip"]
+SOURCES = ["lu2mi_gemm.hip", "lu2mi_lu.hip", "lu2mi_mixed.hip", "lu2mi_batched.hip", "lu2mi_lowp.hip",
+           "lu2mi_lowp_wide.hip", "lu2mi_api.hip"]
 LIB = os.path.join(HERE, "liblu2mi.so")
 DRIVER = os.path.join(HERE, "lu2mi")
 
@@ -22,8 +23,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # MFMA accumulators in ordinary VGPRs: with AGPR accumulators the compiler
 # copies every accumulator across the K-loop back-edge.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-mllvm", "-amdgpu-mfma-vgpr-form=true"]
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# MFMA accumulators in ordinary VGPRs, except where a wave holds more
+# accumulators than the 256 architectural VGPRs (the 16-bit wide tile keeps
+# its 256 in AGPRs)
+VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form=true"]
+AGPR_ACC_SOURCES = {"lu2mi_lowp_wide.hip"}
 
 
 def _run(cmd, **kw):
@@ -41,7 +46,8 @@ def build_library() -> str:
     os.makedirs(os.path.join(CSRC, "_build"), exist_ok=True)
     jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
     with ThreadPoolExecutor(jobs) as ex:
-        list(ex.map(lambda so: _run([HIPCC, *HIPFLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+        list(ex.map(lambda so: _run([HIPCC, *HIPFLAGS, *([] if so[0] in AGPR_ACC_SOURCES else VGPR_FORM),
+                                     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
                                      "-c", os.path.join(CSRC, so[0]), "-o", so[1]]), zip(SOURCES, objs)))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])

@@ -84,6 +84,7 @@ struct lu2mi_ctx {
     int trail_kernel;      // trailing-update tile: 1 direct-to-LDS FP64, 0 register-staged
     int dgemm_as_trailing; // lu2mi_dgemm_dev runs its product as a trailing update
     int strict_sync;       // leaf and back substitution publish with release/acquire
+    int wide16;            // 16-bit updates in 256 x 256 tiles where they fill the device
     hipStream_t stream2;
     hipEvent_t ev_next, ev_panel;
     KernelTimer tcat[LU2MI_T_COUNT];   // per subsystem (LU2MI_T_*)

@@ -16,47 +16,11 @@
 // FP16x2 splits every scaled operand v into hi = fp16(v) and lo = fp16(v-hi)
 // (v-hi is exact in FP32), about 22 significant bits, and accumulates
 // hi*hi + hi*lo + lo*hi: close to FP32 operands at FP16 matrix-core rates.
-#include "lu2mi_internal.h"
-#include "lu2mi_dev.h"
+#include "lu2mi_lowp.h"
 
-typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
-typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
+using namespace lu2mi_lowp;
 
 namespace {
-constexpr int TB = 128;    // scaling block = output tile edge
-constexpr int BKH = 64;    // K per LDS stage (16-bit elements): two 32-deep matrix-core steps
-constexpr int PADH = 8;    // LDS row pad (elements): rows 144 bytes apart, conflict-free fragment reads
-
-template <int FMT> struct H16;
-template <> struct H16<LU2MI_LOWP_FP16> {
-    typedef _Float16 t;
-    typedef h8_t v8;
-    static constexpr int NP = 1;   // operand planes
-    static __device__ __forceinline__ f4_t mfma(v8 a, v8 b, f4_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-    }
-};
-template <> struct H16<LU2MI_LOWP_BF16> {
-    typedef __bf16 t;
-    typedef b8_t v8;
-    static constexpr int NP = 1;
-    static __device__ __forceinline__ f4_t mfma(v8 a, v8 b, f4_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-    }
-};
-
-template <> struct H16<LU2MI_LOWP_FP16X2> {
-    typedef _Float16 t;
-    typedef h8_t v8;
-    static constexpr int NP = 2;   // hi and lo planes
-    static __device__ __forceinline__ f4_t mfma(v8 a, v8 b, f4_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-    }
-};
-
 // power-of-two exponent putting max in [2^13, 2^14); 0 for zero or non-finite
 __device__ __forceinline__ int block_shift(float m)
 {
@@ -267,18 +231,25 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const typename H16<FMT>:
 
 namespace {
 template <int FMT>
-void launch16(lu2mi_ctx *ctx, int M, int N, int K, const float *L, long ldl, const float *U, long ldu, float *C,
+int launch16(lu2mi_ctx *ctx, int M, int N, int K, const float *L, long ldl, const float *U, long ldu, float *C,
               long ldc, int tiles_m, int tiles_n, int *sa, int *sb)
 {
     typedef typename H16<FMT>::t H;
-    const long pa = (long)tiles_m * TB * K, pb = (long)tiles_n * TB * K;
+    const long pa = (long)(tiles_m + 1) * TB * K, pb = (long)(tiles_n + 1) * TB * K;
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(scale_rows16_kernel<FMT>, dim3(tiles_m), dim3(256), 0, st, L, ldl, M, K, (H *)ctx->m_a16, pa,
                        sa);
     hipLaunchKernelGGL(scale_cols16_kernel<FMT>, dim3(tiles_n), dim3(256), 0, st, U, ldu, K, N, (H *)ctx->m_b16, pb,
                        sb);
+    const long tw = (long)((M + 255) / 256) * ((N + 255) / 256);
+    if (ctx->wide16 && tw >= ctx->num_cu) {
+        // the wide tile, when it still gives every CU a workgroup (same bits)
+        return lu2mi_gemm16w_launch(ctx, FMT, ctx->m_a16, ctx->m_b16, M, N, K, pa, pb, sa, sb, tiles_m, tiles_n, C,
+                                    ldc);
+    }
     hipLaunchKernelGGL(gemm16_kernel<FMT>, dim3(tiles_m * tiles_n), dim3(256), 0, st, (const H *)ctx->m_a16,
                        (const H *)ctx->m_b16, M, N, K, pa, pb, sa, sb, C, ldc);
+    return LU2MI_OK;
 }
 }  // namespace
 
@@ -294,16 +265,18 @@ int lu2mi_update16(lu2mi_ctx *ctx, int fmt, int M, int N, int K, const float *L,
     }
     const int tiles_m = (M + TB - 1) / TB, tiles_n = (N + TB - 1) / TB;
     const size_t np = (fmt == LU2MI_LOWP_FP16X2) ? 2 : 1;
-    LU2MI_TRY(lu2mi_dev_buf(&ctx->m_a16, &ctx->m_a16_bytes, np * tiles_m * TB * K * 2));
-    LU2MI_TRY(lu2mi_dev_buf(&ctx->m_b16, &ctx->m_b16_bytes, np * tiles_n * TB * K * 2));
+    // operand copies padded to whole 128-row blocks, plus one more block so
+    // that a wide tile's second half past the last block reads inside them
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->m_a16, &ctx->m_a16_bytes, np * (tiles_m + 1) * TB * K * 2));
+    LU2MI_TRY(lu2mi_dev_buf(&ctx->m_b16, &ctx->m_b16_bytes, np * (tiles_n + 1) * TB * K * 2));
     LU2MI_TRY(lu2mi_dev_buf(&ctx->m_s16, &ctx->m_s16_bytes, sizeof(int) * (size_t)(tiles_m + tiles_n)));
     int *sa = (int *)ctx->m_s16, *sb = sa + tiles_m;
     if (fmt == LU2MI_LOWP_FP16)
-        launch16<LU2MI_LOWP_FP16>(ctx, M, N, K, L, ldl, U, ldu, C, ldc, tiles_m, tiles_n, sa, sb);
+        LU2MI_TRY(launch16<LU2MI_LOWP_FP16>(ctx, M, N, K, L, ldl, U, ldu, C, ldc, tiles_m, tiles_n, sa, sb));
     else if (fmt == LU2MI_LOWP_BF16)
-        launch16<LU2MI_LOWP_BF16>(ctx, M, N, K, L, ldl, U, ldu, C, ldc, tiles_m, tiles_n, sa, sb);
+        LU2MI_TRY(launch16<LU2MI_LOWP_BF16>(ctx, M, N, K, L, ldl, U, ldu, C, ldc, tiles_m, tiles_n, sa, sb));
     else
-        launch16<LU2MI_LOWP_FP16X2>(ctx, M, N, K, L, ldl, U, ldu, C, ldc, tiles_m, tiles_n, sa, sb);
+        LU2MI_TRY(launch16<LU2MI_LOWP_FP16X2>(ctx, M, N, K, L, ldl, U, ldu, C, ldc, tiles_m, tiles_n, sa, sb));
     LU2MI_CHECK(hipGetLastError());
     return LU2MI_OK;
 }

@@ -175,6 +175,10 @@ __device__ __forceinline__ bool better(double v, int r, double bv, int br)
 }
 }  // namespace
 
+// maximum that keeps a NaN (fmax drops it): a NaN solution must not pass as
+// a small residual
+__device__ __forceinline__ double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
 template <typename T>
 struct LeafArgs {
     T *A;
@@ -990,18 +994,18 @@ __global__ __launch_bounds__(256) void resid_final_kernel(const double *A, long 
     const int t = threadIdx.x;
     double v[6] = {0, 0, 0, 0, 0, 0};
     for (int r = t; r < n; r += 256) {
-        v[0] = fmax(v[0], rerr[r]);
-        v[1] = fmax(v[1], rabs[r]);
-        v[2] = fmax(v[2], cabs[r]);
-        v[3] = fmax(v[3], fabs(x[r]));
+        v[0] = nan_max(v[0], rerr[r]);
+        v[1] = nan_max(v[1], rabs[r]);
+        v[2] = nan_max(v[2], cabs[r]);
+        v[3] = nan_max(v[3], fabs(x[r]));
         v[4] += fabs(x[r]);
-        v[5] = fmax(v[5], fabs(A[(long)r * lda + n]));
+        v[5] = nan_max(v[5], fabs(A[(long)r * lda + n]));
     }
     for (int k = 0; k < 6; k++) sh[k][t] = v[k];
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
         if (t < st)
-            for (int k = 0; k < 6; k++) sh[k][t] = (k == 4) ? sh[k][t] + sh[k][t + st] : fmax(sh[k][t], sh[k][t + st]);
+            for (int k = 0; k < 6; k++) sh[k][t] = (k == 4) ? sh[k][t] + sh[k][t + st] : nan_max(sh[k][t], sh[k][t + st]);
         __syncthreads();
     }
     if (t < 6) out[t] = sh[t][0];

@@ -132,13 +132,14 @@ __global__ __launch_bounds__(256) void multidot_kernel(const double *x, const do
     if (m == 0) {
         double mx = 0.0, ss = 0.0;
         for (long e = lo + t; e < hi; e += 256) {
-            mx = fmax(mx, fabs(x[e]));
+            const double ax = fabs(x[e]);
+            mx = (ax > mx || ax != ax) ? ax : mx;   // keeps a NaN
             ss = fma(x[e], x[e], ss);
         }
         sh[t] = mx;
         __syncthreads();
         for (int st = 128; st > 0; st >>= 1) {
-            if (t < st) sh[t] = fmax(sh[t], sh[t + st]);
+            if (t < st) sh[t] = (sh[t + st] > sh[t] || sh[t + st] != sh[t + st]) ? sh[t + st] : sh[t];
             __syncthreads();
         }
         if (t == 0) part[blockIdx.x] = sh[0];
@@ -246,7 +247,7 @@ struct Mixed {
         LU2MI_CHECK(hipStreamSynchronize(s()));
         double mx = 0, ss = 0;
         for (int g = 0; g < NPART; g++) {
-            mx = fmax(mx, h[g]);
+            mx = (h[g] > mx || h[g] != h[g]) ? h[g] : mx;   // keeps a NaN
             ss += h[NPART + g];
         }
         a[0] = mx;

@@ -131,6 +131,25 @@ def test_residual_dev_matches_oracle(ctx, oracle, n, seed, perturb):
     np.testing.assert_allclose(got[9], got[0] / (eps_lu2 * got[1] * got[3] * n), rtol=1e-14)
 
 
+def test_residual_dev_keeps_nan(ctx, oracle):
+    """A solution with a NaN must not pass as a small residual: the maxima
+    keep NaN (fmax would drop it)."""
+    n = 1000
+    lda = n + 16
+    a0 = oracle.randomsetmat(n, 2)
+    x = np.linalg.solve(a0[:, :n], a0[:, n])
+    x[17] = np.nan
+    dA, dx = ctx.alloc(8 * n * lda), ctx.alloc(8 * n)
+    try:
+        dA.upload(a0)
+        dx.upload(x)
+        got = ctx.residual_dev(n, dA, lda, dx)
+    finally:
+        dA.free()
+        dx.free()
+    assert np.isnan(got[0]) and np.isnan(got[3]) and np.isnan(got[6])
+
+
 # ------------------------------------------------- full-size properties
 N3, NB3 = 65536, 2048
 

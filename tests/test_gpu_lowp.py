@@ -234,3 +234,58 @@ def test_mixed_subpanel_bit_identical(ctx, prec):
     assert np.array_equal(xs[0], xs[1])
     assert infos[0]["iterations"] == infos[1]["iterations"]
     assert infos[0]["resid_hpl2_initial"] == infos[1]["resid_hpl2_initial"]
+
+
+@pytest.mark.parametrize("fmt", [L.LOWP_FP16, L.LOWP_BF16, L.LOWP_FP16X2])
+def test_update16_wide_tile_bit_identical(ctx, fmt):
+    """The 256 x 256 tile (128 x 64 per wave, inside one scaling-block pair)
+    performs each accumulator's matrix-core ops in the 128 x 128 tile's
+    order: identical bits, edge tiles included (M, N not multiples of 256,
+    the last wide tile reaching past the last scaling block)."""
+    m, n, k = 4096 + 100, 4096 + 37, 192
+    rng = np.random.default_rng(fmt)
+    l = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    u = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    l[:128] *= np.float32(2.0 ** 12)
+    u[:, 256:384] *= np.float32(2.0 ** -9)
+    c = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    out = []
+    for wide in (0, 1):
+        dL, dU, dC = ctx.alloc(l.nbytes), ctx.alloc(u.nbytes), ctx.alloc(c.nbytes)
+        dL.upload(l)
+        dU.upload(u)
+        dC.upload(c)
+        ctx.set_option(L.OPT_WIDE16, wide)
+        try:
+            ctx.gemm16_dev(fmt, m, n, k, dL, k, dU, n, dC, n)
+        finally:
+            ctx.set_option(L.OPT_WIDE16, 0)   # the library default
+        out.append(dC.download(np.float32, m * n))
+        for d in (dL, dU, dC):
+            d.free()
+    assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("prec", [L.PREC_FP16X2, L.PREC_BF16])
+def test_mixed_wide_tile_bit_identical(ctx, prec):
+    """A mixed solve whose trailing updates use the wide tile gives the same
+    factors and solution as with the 128 x 128 tile."""
+    n, nb = 8192, 512
+    lda = n + 16
+    dA = ctx.alloc(8 * n * lda)
+    ctx.randomsetmat(n, 4, dA, lda)
+    ctx.make_diag_dominant(n, dA, lda)
+    ctx.set_option(L.OPT_PIVMODE, L.PIV_GEPP)
+    xs, infos = [], []
+    for wide in (0, 1):
+        dx = ctx.alloc(8 * n)
+        ctx.set_option(L.OPT_WIDE16, wide)
+        try:
+            infos.append(ctx.solve_mixed(n, dA, lda, nb, dx, prec=prec, maxiter=30, target=1.0))
+        finally:
+            ctx.set_option(L.OPT_WIDE16, 0)
+        xs.append(dx.download(np.float64, n))
+        dx.free()
+    dA.free()
+    assert np.array_equal(xs[0], xs[1])
+    assert infos[0]["iterations"] == infos[1]["iterations"]

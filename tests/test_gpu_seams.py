@@ -169,8 +169,11 @@ def test_lumcolumn_host_dropin(lib, oracle, n, nb, seed):
     assert np.array_equal(a[:, : n + 1], fd[:, : n + 1])
 
 
-# more than 32768 rows: the leaf holds four rows per thread
-@pytest.mark.parametrize("n,m,i", [(300, 16, 0), (1000, 16, 64), (777, 32, 100), (40000, 16, 0), (41000, 16, 1000)])
+# more than 32768 rows: the leaf holds four rows per thread; more than 65536:
+# more than 64 workgroups, so fewer threads poll each exchange slot (three,
+# two, and one at 128 workgroups)
+@pytest.mark.parametrize("n,m,i", [(300, 16, 0), (1000, 16, 64), (777, 32, 100), (40000, 16, 0), (41000, 16, 1000),
+                                   (70000, 16, 0), (100000, 16, 0), (130100, 16, 0)])
 def test_cm_column_decomposition_recursive(lib, oracle, n, m, i):
     """lu2tlib.c:1130 on the column-major awork of lu2.c:1072-1085: same
     pivots as the oracle's restatement, values to rounding (lu2's rule)."""
