@@ -255,6 +255,87 @@ def host_gemm_probe(m: int, n: int, k: int) -> int:
     return 0
 
 
+def short_kernel_name(name: str) -> str:
+    """`void leaf_kernel<double, 16, 4>(double*, ...)` -> `leaf_kernel<double, 16, 4>`."""
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    name = name.replace("(anonymous namespace)::", "")
+    depth = 0
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0 and i > 0:
+            return name[:i]
+    return name
+
+
+def pmc_table(paths: list, copy_bytes: float | None) -> int:
+    """Per-kernel totals of rocprofv3 --pmc counter files (run_counter_collection.csv,
+    one per run of the same program, dispatches in the same order): dispatches,
+    summed duration, and for the counters present FETCH_SIZE / WRITE_SIZE in GB
+    (scaled so that the largest device-to-device copy of the run, copy_bytes
+    long, reads and writes copy_bytes), the HBM rate over the summed duration,
+    and the matrix-core busy fraction SQ_VALU_MFMA_BUSY_CYCLES over
+    GRBM_GUI_ACTIVE / 8 XCDs x 1024 SIMDs.  Prints a markdown table."""
+    import csv
+    from collections import defaultdict
+
+    ctr = defaultdict(lambda: defaultdict(float))     # kernel -> counter -> sum
+    disp = defaultdict(int)
+    ns = defaultdict(int)
+    copy_val = {}
+    for k, p in enumerate(paths):
+        seen = set()
+        with open(p, newline="") as f:
+            for r in csv.DictReader(f):
+                name = short_kernel_name(r["Kernel_Name"])
+                cname, val = r["Counter_Name"], float(r["Counter_Value"])
+                ctr[name][cname] += val
+                if name == "__amd_rocclr_copyBuffer":
+                    copy_val[cname] = max(copy_val.get(cname, 0.0), val)
+                did = r["Dispatch_Id"]
+                if k == 0 and did not in seen:
+                    seen.add(did)
+                    disp[name] += 1
+                    ns[name] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    scale = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        scale[c] = (copy_bytes / 1024.0 / copy_val[c]) if (copy_bytes and copy_val.get(c)) else 1.0
+    have = {c for d in ctr.values() for c in d}
+    hdr = ["kernel", "dispatches", "ms"]
+    if "FETCH_SIZE" in have:
+        hdr.append("fetched GB")
+    if "WRITE_SIZE" in have:
+        hdr.append("written GB")
+    if have & {"FETCH_SIZE", "WRITE_SIZE"}:
+        hdr.append("HBM GB/s")
+    mfma = {"SQ_VALU_MFMA_BUSY_CYCLES", "GRBM_GUI_ACTIVE"} <= have
+    if mfma:
+        hdr.append("MFMA busy %")
+    print("| " + " | ".join(hdr) + " |")
+    print("|" + "---|" + "---:|" * (len(hdr) - 1))
+    for name in sorted(ctr, key=lambda x: -ns[x]):
+        d, ms = ctr[name], ns[name] / 1e6
+        row = [f"`{name}`", str(disp[name]), f"{ms:.2f}"]
+        gb = 0.0
+        for c, col in (("FETCH_SIZE", "fetched GB"), ("WRITE_SIZE", "written GB")):
+            if col in hdr:
+                v = d.get(c, 0.0) * 1024.0 * scale[c] / 1e9
+                gb += v
+                row.append(f"{v:.3f}")
+        if "HBM GB/s" in hdr:
+            row.append(f"{gb / (ms / 1e3):.0f}" if ms > 0 else "-")
+        if mfma:
+            den = d.get("GRBM_GUI_ACTIVE", 0.0) / 8.0 * 1024.0
+            row.append(f"{100.0 * d.get('SQ_VALU_MFMA_BUSY_CYCLES', 0.0) / den:.1f}" if den > 0 else "-")
+        print("| " + " | ".join(row) + " |")
+    print(f"\nscale: FETCH_SIZE x {scale['FETCH_SIZE']:.3f}, WRITE_SIZE x {scale['WRITE_SIZE']:.3f}")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,8 +374,14 @@ def main() -> int:
                     help="time a host-memory product through the mydgemm seam (PCIe staging)")
     ap.add_argument("--trail-kernel", type=int, default=1, choices=(0, 1),
                     help="trailing-update DGEMM tile: 1 direct-to-LDS (default), 0 register-staged")
+    ap.add_argument("--pmc-table", nargs="+", metavar="CSV", default=None,
+                    help="summarise rocprofv3 --pmc counter files per kernel (no GPU use)")
+    ap.add_argument("--pmc-copy-bytes", type=float, default=None,
+                    help="bytes of the run's largest device-to-device copy (counter calibration)")
     args = ap.parse_args()
 
+    if args.pmc_table:
+        return pmc_table(args.pmc_table, args.pmc_copy_bytes)
     if args.gemm_probe:
         return gemm_probe(*args.gemm_probe, args.probe_reps, args.trail_kernel)
     if args.host_gemm:

@@ -51,11 +51,14 @@ extern "C" {
                                   by one persistent launch with 16-wide leaves and grid
                                   barriers (default 0 = one launch per leaf and per GEMM,
                                   which measured faster; results identical either way) */
-#define LU2MI_OPT_TRAIL_KERNEL 6  /* trailing-update DGEMM tile: 1 = operands loaded straight
-                                     into LDS (default), 0 = staged through registers; the
-                                     two sum each product's K terms in different orders */
+#define LU2MI_OPT_TRAIL_KERNEL 6  /* FP64 trailing-update and U12 DGEMM tile: 1 = operands
+                                     loaded straight into LDS (default), 0 = staged through
+                                     registers; the two sum each product's K terms in
+                                     different orders */
 #define LU2MI_OPT_DGEMM_TRAILING 7 /* 1: lu2mi_dgemm_dev runs its product with the
-                                     trailing-update kernel (to measure that kernel alone) */
+                                     trailing-update kernel (to measure that kernel alone);
+                                     2: the same, with A taken as lower triangular (the U12
+                                     product's form: entries above the diagonal are not read) */
 #define LU2MI_OPT_STRICT_SYNC 8   /* 1: the panel leaf and the back substitution publish their
                                      cross-workgroup data with agent-scope release/acquire
                                      (an L2 writeback and invalidate per publication); default

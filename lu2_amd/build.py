@@ -29,6 +29,12 @@ HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
 # its 256 in AGPRs)
 VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form=true"]
 AGPR_ACC_SOURCES = {"lu2mi_lowp_wide.hip"}
+# Kernels that must not spill registers to scratch: the direct-to-LDS DGEMM
+# tile counts on its loads being the only vector-memory operations in flight
+# at its barriers (a spilling variant gave wrong products).  The build fails
+# if the compiler reports scratch use for any kernel in these files.
+NOSPILL_SOURCES = {"lu2mi_gemm.hip"}
+RESOURCE_REMARKS = ["-Rpass-analysis=kernel-resource-usage"]
 
 
 def _run(cmd, **kw):
@@ -39,16 +45,39 @@ def _run(cmd, **kw):
     return r
 
 
+def _check_no_scratch(src: str, remarks: str) -> None:
+    """Fail when the compiler's resource-usage remarks give any kernel of src
+    a nonzero ScratchSize (register spills)."""
+    name, seen = None, 0
+    for line in remarks.splitlines():
+        if "Function Name:" in line:
+            name = line.split("Function Name:", 1)[1].split()[0]
+        elif "ScratchSize [bytes/lane]:" in line:
+            seen += 1
+            size = int(line.split("ScratchSize [bytes/lane]:", 1)[1].split()[0])
+            if size:
+                raise RuntimeError(f"{src}: kernel {name} uses {size} bytes/lane of scratch (register spills)")
+    if not seen:
+        raise RuntimeError(f"{src}: no resource-usage remarks from the compiler")
+
+
 def build_library() -> str:
     from concurrent.futures import ThreadPoolExecutor
 
     objs = [os.path.join(CSRC, "_build", src.replace(".hip", ".o")) for src in SOURCES]
     os.makedirs(os.path.join(CSRC, "_build"), exist_ok=True)
     jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    def compile_one(so):
+        src, obj = so
+        r = _run([HIPCC, *HIPFLAGS, *([] if src in AGPR_ACC_SOURCES else VGPR_FORM),
+                  *(RESOURCE_REMARKS if src in NOSPILL_SOURCES else []),
+                  "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+                  "-c", os.path.join(CSRC, src), "-o", obj])
+        if src in NOSPILL_SOURCES:
+            _check_no_scratch(src, r.stderr)
+
     with ThreadPoolExecutor(jobs) as ex:
-        list(ex.map(lambda so: _run([HIPCC, *HIPFLAGS, *([] if so[0] in AGPR_ACC_SOURCES else VGPR_FORM),
-                                     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-                                     "-c", os.path.join(CSRC, so[0]), "-o", so[1]]), zip(SOURCES, objs)))
+        list(ex.map(compile_one, zip(SOURCES, objs)))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -160,7 +160,7 @@ extern "C" int lu2mi_set_option(lu2mi_ctx *ctx, int opt, int value)
         ctx->trail_kernel = value;
         return LU2MI_OK;
     case LU2MI_OPT_DGEMM_TRAILING:
-        ctx->dgemm_as_trailing = value ? 1 : 0;
+        ctx->dgemm_as_trailing = (value == 2) ? 2 : value ? 1 : 0;
         return LU2MI_OK;
     case LU2MI_OPT_STRICT_SYNC:
         ctx->strict_sync = value ? 1 : 0;
@@ -279,7 +279,7 @@ extern "C" int lu2mi_dgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, double alpha
     GemmDesc g;
     g.M = m; g.N = n; g.K = k; g.batch = 1; g.alpha = alpha; g.beta = beta;
     g.A = dA; g.lda = lda; g.sA = 0; g.B = dB; g.ldb = ldb; g.sB = 0; g.C = dC; g.ldc = ldc; g.sC = 0;
-    g.a_lower = 0;
+    g.a_lower = ctx->dgemm_as_trailing == 2;   // the U12 product's form: A lower triangular
     if (ctx->dgemm_as_trailing) g.tag = ctx->trail_kernel ? 1 : 2;
     LU2MI_TRY(lu2mi_gemm_f64(g, ctx->stream));
     LU2MI_CHECK(hipStreamSynchronize(ctx->stream));

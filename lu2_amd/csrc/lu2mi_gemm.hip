@@ -78,6 +78,67 @@ __global__ __launch_bounds__(lu2mi_gemm2::NT, 2) void gemm2_kernel(GemmDesc g, i
     else G2::tile<true, EPI>(g, A, B, C, m0, n0, g.K, sm);
 }
 
+// The U12 product C = Linv * rows (lu2.c:511-512): A lower triangular, so row
+// tile r stops at the diagonal and carries r+1 units of K work.  Workgroups
+// are placed on the CUs in a fixed rotation, so tiles of unequal length leave
+// CUs idle (a 2048-row product took as long as with the whole K range): each
+// workgroup here takes row tiles p and T-1-p of one column (T+1 units
+// whatever p), the grid covering ceil(T/2) row pairs.  The tiles are the
+// single-tile kernels' (same bits).
+template <typename T>
+__global__ __launch_bounds__(NT) void gemm_lower_kernel(GemmDesc g, int tiles_m, int tiles_n)
+{
+    constexpr int BKT = Mfma<T>::BK;
+    __shared__ T As[2][BM * (BKT + 1)];
+    __shared__ T Bs[2][BKT * BN];
+    const int slots_m = (tiles_m + 1) / 2;
+    int tp, tn;
+    tile_coords(xcd_remap(blockIdx.x, slots_m * tiles_n), slots_m, tiles_n, tp, tn);
+    const T *A = (const T *)g.A;
+    const T *B = (const T *)g.B;
+    T *C = (T *)g.C;
+    const int n0 = tn * BN;
+    auto one = [&](int tm) {
+        const int m0 = tm * BM;
+        const int K = min(g.K, m0 + BM);
+        if ((m0 + BM <= g.M) && (n0 + BN <= g.N) && (K % BKT == 0))
+            gemm_tile<T, false, EPI_SET>(g, A, B, C, m0, n0, K, As, Bs);
+        else
+            gemm_tile<T, true, EPI_SET>(g, A, B, C, m0, n0, K, As, Bs);
+    };
+    one(tp);
+    if (tiles_m - 1 - tp != tp) {
+        __syncthreads();   // every wave is done with the first tile's LDS
+        one(tiles_m - 1 - tp);
+    }
+}
+
+__global__ __launch_bounds__(lu2mi_gemm2::NT, 2) void gemm2_lower_kernel(GemmDesc g, int tiles_m, int tiles_n)
+{
+    namespace G2 = lu2mi_gemm2;
+    __shared__ G2::Smem sm;
+    const int slots_m = (tiles_m + 1) / 2;
+    int tp, tn;
+    tile_coords(xcd_remap(blockIdx.x, slots_m * tiles_n), slots_m, tiles_n, tp, tn);
+    const double *A = (const double *)g.A;
+    const double *B = (const double *)g.B;
+    double *C = (double *)g.C;
+    const int n0 = tn * G2::BN;
+    auto one = [&](int tm) {
+        const int m0 = tm * G2::BM;
+        const int K = min(g.K, m0 + G2::BM);
+        if ((m0 + G2::BM <= g.M) && (n0 + G2::BN <= g.N) && (K % G2::BK == 0))
+            G2::tile<false, EPI_SET>(g, A, B, C, m0, n0, K, sm);
+        else
+            G2::tile<true, EPI_SET>(g, A, B, C, m0, n0, K, sm);
+    };
+    one(tp);
+    if (tiles_m - 1 - tp != tp) {
+        __syncthreads();   // every wave is done with the first tile's LDS
+        one(tiles_m - 1 - tp);
+    }
+}
+
 // ---------------------------------------------------------------------------
 static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
@@ -122,6 +183,16 @@ static int launch_gemm(const GemmDesc &g, hipStream_t s)
             hipLaunchKernelGGL((gemm64_kernel<T, EPI_SET>), grid64, block64, 0, s, g, tm64, tn64);
         else
             hipLaunchKernelGGL((gemm64_kernel<T, EPI_GENERAL>), grid64, block64, 0, s, g, tm64, tn64);
+        LU2MI_CHECK(hipGetLastError());
+        return LU2MI_OK;
+    }
+    if (g.a_lower && epi == EPI_SET && vec && g.batch == 1) {
+        // the U12 product: pairs of row tiles per workgroup
+        dim3 gridp((unsigned)((long)((tiles_m + 1) / 2) * tiles_n)), blockp(NT);
+        if (sizeof(T) == 8 && g.tag == 1 && aligned16(g.C) && g.ldc % 2 == 0)
+            hipLaunchKernelGGL(gemm2_lower_kernel, gridp, blockp, 0, s, g, tiles_m, tiles_n);
+        else
+            hipLaunchKernelGGL(gemm_lower_kernel<T>, gridp, blockp, 0, s, g, tiles_m, tiles_n);
         LU2MI_CHECK(hipGetLastError());
         return LU2MI_OK;
     }

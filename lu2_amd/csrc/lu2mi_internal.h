@@ -35,9 +35,10 @@ struct GemmDesc {
     const void *B = nullptr; long ldb = 0; long sB = 0;
     void *C = nullptr; long ldc = 0; long sC = 0;
     int a_lower = 0;
-    // 0: any product; 1: trailing update in the direct-to-LDS FP64 tile
-    // (gemm2_kernel); 2: trailing update in the register-staged tile.  Both
-    // trailing forms are kernel symbols of their own (for profiling).
+    // 0: any product; 1: trailing update or U12 product in the direct-to-LDS
+    // FP64 tile (gemm2_kernel); 2: trailing update in the register-staged
+    // tile.  Both trailing forms are kernel symbols of their own (for
+    // profiling).
     int tag = 0;
 };
 int lu2mi_gemm_f64(const GemmDesc &g, hipStream_t s);

@@ -1404,7 +1404,8 @@ struct Factor {
     {
         int ts;
         LU2MI_TRY(tbegin(LU2MI_T_U12, ts));
-        LU2MI_TRY(gemm_call<T>(ctx, nb, N, nb, 1.0, linv, nb, wb, ldw, 0.0, A + (long)i * lda + c0, lda, 1));
+        LU2MI_TRY(gemm_call<T>(ctx, nb, N, nb, 1.0, linv, nb, wb, ldw, 0.0, A + (long)i * lda + c0, lda, 1,
+                               (sizeof(T) == 8 && ctx->trail_kernel) ? 1 : 0));
         return tend(LU2MI_T_U12, ts, (double)nb * nb * N, 0.0);
     }
 
