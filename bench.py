@@ -183,20 +183,22 @@ def batched_config(ctx, seed: int) -> dict:
     return out
 
 
-def gemm_probe(m: int, n: int, k: int, reps: int, trail_kernel: int) -> int:
+def gemm_probe(m: int, n: int, k: int, reps: int, trail_kernel: int, lower: bool = False) -> int:
     """The trailing-update product on its own: C[m x n] -= A[m x k] B[k x n]
     inside one row-major matrix (A = rows k.., columns 0..k; B = rows 0..k,
     columns k..; C = rows k.., columns k..), as the factorization lays it
     out, with the n=65536 factorization's leading dimension 65552 (or
     m + k + 16 if larger), after a 1 GiB device-to-device copy that
-    calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters.  Prints one JSON
-    line with the mean time and rate over `reps` launches."""
+    calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters.  With lower, the
+    U12 product's form instead: C = A B with A lower triangular (entries
+    above its diagonal are not read), whose flops are 2 sum_r min(r+1, k) n.
+    Prints one JSON line with the median time and rate over `reps` launches."""
     import lu2_amd as L
 
     lda = max(m + k + 16, 65552)
     ctx = L.Context(0)
     ctx.set_option(L.OPT_TRAIL_KERNEL, trail_kernel)
-    ctx.set_option(L.OPT_DGEMM_TRAILING, 1)
+    ctx.set_option(L.OPT_DGEMM_TRAILING, 2 if lower else 1)
     dX = ctx.alloc(8 * (m + k) * lda)
     ctx.randomsetmat(m + k, 1, dX, lda)
     cal = ctx.alloc(1 << 30)
@@ -207,17 +209,51 @@ def gemm_probe(m: int, n: int, k: int, reps: int, trail_kernel: int) -> int:
     for _ in range(reps):
         L._lib.lu2mi_synchronize(ctx.handle)
         t0 = time.perf_counter()
-        ctx.dgemm_dev(m, n, k, -1.0, dX, lda, dX, lda, 1.0, dX, lda, offs=(offA, offB, offC))
+        if lower:
+            ctx.dgemm_dev(m, n, k, 1.0, dX, lda, dX, lda, 0.0, dX, lda, offs=(offA, offB, offC))
+        else:
+            ctx.dgemm_dev(m, n, k, -1.0, dX, lda, dX, lda, 1.0, dX, lda, offs=(offA, offB, offC))
         times.append(time.perf_counter() - t0)
     for b in (dX, cal, tmp):
         b.free()
     ctx.close()
     t = sorted(times)[len(times) // 2]
-    fl = 2.0 * m * n * k
-    print(json.dumps({"probe": "trailing-update DGEMM", "kernel": TRAIL_KERNEL_NAME[trail_kernel], "M": m, "N": n, "K": k, "lda": lda, "reps": reps,
+    fl = 2.0 * n * sum(min(r + 1, k) for r in range(m)) if lower else 2.0 * m * n * k
+    kname = ("gemm2_lower_kernel (U12 form, row tiles paired)" if trail_kernel else
+             "gemm_lower_kernel<double> (U12 form, register-staged)") if lower else TRAIL_KERNEL_NAME[trail_kernel]
+    print(json.dumps({"probe": "U12 product" if lower else "trailing-update DGEMM", "kernel": kname,
+                      "M": m, "N": n, "K": k, "lda": lda, "reps": reps,
                       "median_ms": t * 1e3, "tflops": fl / t / 1e12,
                       "frac_of_fp64_peak": fl / t / 1e12 / FP64_MFMA_PEAK_TFLOPS,
-                      "algorithmic_bytes": 8.0 * (2 * m * n + m * k + k * n)}), flush=True)
+                      "algorithmic_bytes": 8.0 * ((1 if lower else 2) * m * n + m * k + k * n)}), flush=True)
+    return 0
+
+
+def leaf_probe(rows: int, reps: int) -> int:
+    """One 16-column panel leaf on its own: lu2's column-major panel seam
+    (lu2tlib.c:1130 cm_column_decomposition_recursive, m = 16, the width
+    lu2.c:1082 uses) on a random rows x 16 panel, which the library factors
+    with one leaf launch (plus its host<->device copies).  For counter runs
+    of the leaf kernel; prints the wall time of each call."""
+    import ctypes
+
+    import numpy as np
+
+    import lu2_amd as L
+
+    lib = ctypes.CDLL(L.LIB_PATH)
+    lib.cm_column_decomposition_recursive.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                      ctypes.c_void_p, ctypes.c_int]
+    rng = np.random.default_rng(rows)
+    a0 = rng.uniform(-0.5, 0.5, (16, rows))
+    pv = np.zeros(16, np.int32)
+    for r in range(reps):
+        a = a0.copy()
+        t0 = time.perf_counter()
+        lib.cm_column_decomposition_recursive(rows, a.ctypes.data, 16, pv.ctypes.data, 0)
+        print(json.dumps({"probe": "panel leaf (16 columns)", "rows": rows, "call": r,
+                          "wall_ms": (time.perf_counter() - t0) * 1e3,
+                          "leaf_bytes": 2.0 * rows * 16 * 8}), flush=True)
     return 0
 
 
@@ -351,7 +387,8 @@ def main() -> int:
                     help="also time the CPU oracle at n=32768 nb=512 (minutes)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=1)
-    ap.add_argument("--mixed-nb", type=int, default=2048)
+    ap.add_argument("--mixed-nb", type=int, default=None,
+                    help="block size of both mixed solves (default: 2048 FP16x2, 1024 FP32)")
     ap.add_argument("--no-lowp", action="store_true")
     ap.add_argument("--lowp-n", type=int, default=98304)
     ap.add_argument("--lowp-nb", type=int, default=2048)
@@ -370,6 +407,10 @@ def main() -> int:
     ap.add_argument("--gemm-probe", type=int, nargs=3, metavar=("M", "N", "K"), default=None,
                     help="time the trailing-update DGEMM alone at this shape (counter runs)")
     ap.add_argument("--probe-reps", type=int, default=1)
+    ap.add_argument("--probe-lower", action="store_true",
+                    help="--gemm-probe of the U12 product's form (A lower triangular, C = A B)")
+    ap.add_argument("--leaf-probe", type=int, default=None, metavar="ROWS",
+                    help="one 16-column panel leaf of ROWS rows through the column-major seam")
     ap.add_argument("--host-gemm", type=int, nargs=3, metavar=("M", "N", "K"), default=None,
                     help="time a host-memory product through the mydgemm seam (PCIe staging)")
     ap.add_argument("--trail-kernel", type=int, default=1, choices=(0, 1),
@@ -383,9 +424,11 @@ def main() -> int:
     if args.pmc_table:
         return pmc_table(args.pmc_table, args.pmc_copy_bytes)
     if args.gemm_probe:
-        return gemm_probe(*args.gemm_probe, args.probe_reps, args.trail_kernel)
+        return gemm_probe(*args.gemm_probe, args.probe_reps, args.trail_kernel, args.probe_lower)
     if args.host_gemm:
         return host_gemm_probe(*args.host_gemm)
+    if args.leaf_probe:
+        return leaf_probe(args.leaf_probe, args.probe_reps)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the LU on a single MI355X "
               "(no multi-GPU path); run with --gpus 1", file=sys.stderr)
@@ -457,7 +500,14 @@ def main() -> int:
         dxm = ctx.alloc(8 * n)
         mixed = {}
         try:
-            for name, prec, mnb in (("fp16x2", L.PREC_FP16X2, args.mixed_nb), ("fp32", L.PREC_FP32, args.mixed_nb)):
+            # block sizes per form (sweeps at n=65536): the FP16x2 updates round
+            # their operands once per block update, so fewer, wider blocks keep
+            # the factors accurate enough for 3 refinement steps (nb=1024: 8);
+            # the FP32 form converges in 4 either way and nb=1024 shortens its
+            # panels
+            nb16 = args.mixed_nb or 2048
+            nb32 = args.mixed_nb or 1024
+            for name, prec, mnb in (("fp16x2", L.PREC_FP16X2, nb16), ("fp32", L.PREC_FP32, nb32)):
                 tms = []
                 for w in range(args.warmup + args.mixed_steps):
                     L._lib.lu2mi_synchronize(ctx.handle)
