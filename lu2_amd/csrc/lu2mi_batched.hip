@@ -202,7 +202,7 @@ __device__ __forceinline__ void batch_trsm(T *W, int ldw, int j0, int w, int N2,
 // row now at row i.
 template <typename T, int NT, int WP>
 __device__ __forceinline__ void batch_factor(T *__restrict__ W, int ldw, int n, int nc, T *P, T *Xi, int *piv,
-                                             int *perm, double *red_v, int *red_i, int *map, int *lst)
+                                             int *perm, double *red_v, int *red_i)
 {
     constexpr int PP = WP + 1, NW = NT / 64;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -259,77 +259,36 @@ __device__ __forceinline__ void batch_factor(T *__restrict__ W, int ldw, int n, 
             const int r = idx / w, c = idx - r * w;
             W[(long)(j0 + r) * ldw + j0 + c] = P[r * PP + c];
         }
-        // The panel's w exchanges on every other column, in step order.  Their
-        // net effect on a column is a permutation of at most 2w rows (the block
-        // rows and the pivot rows): map[r] = the row whose value ends at r,
-        // found by replaying the exchanges on row indices; then each column's
-        // moved values are all read before any is written, instead of w
-        // dependent exchanges through memory.
-        for (int jj = t; jj < w; jj += NT) {
-            map[j0 + jj] = j0 + jj;
-            map[piv[j0 + jj]] = piv[j0 + jj];
+        // X = L11^-1 for the TRSM, in Xi: start from I, then row k (final once
+        // step k begins) eliminates column k from the rows below it
+        for (int idx = t; idx < WP * WP; idx += NT) {
+            const int i = idx / WP, j = idx - i * WP;
+            Xi[i * PP + j] = (i == j) ? T(1) : T(0);
         }
-        __syncthreads();
-        if (t == 0) {
+        // the panel's swaps on all other columns, in step order
+        const int no = nc - w;
+        for (int cc = t; cc < no; cc += NT) {
+            const int col = cc < j0 ? cc : cc + w;
             for (int jj = 0; jj < w; jj++) {
                 const int p = piv[j0 + jj];
-                const int x = map[j0 + jj];
-                map[j0 + jj] = map[p];
-                map[p] = x;
-            }
-            int cnt = 0;   // positions whose value changes, each once
-            for (int k = 0; k < 2 * w; k++) {
-                const int r = k < w ? j0 + k : piv[j0 + k - w];
-                if (map[r] != r) {
-                    lst[cnt] = r;
-                    lst[2 * WP + cnt] = map[r];
-                    map[r] = r;   // listed
-                    cnt++;
+                if (p != j0 + jj) {
+                    T *x = W + (long)(j0 + jj) * ldw + col;
+                    T *y = W + (long)p * ldw + col;
+                    const T v = *x;
+                    *x = *y;
+                    *y = v;
                 }
-            }
-            lst[4 * WP] = cnt;
-        }
-        // X = L11^-1 for the TRSM, in Xi, column j by lane j of wave 0: row i
-        // of X (i > j) is -sum_{k=j}^{i-1} L[i][k] X[k][j], the k terms in
-        // increasing order (the order of the row-elimination form)
-        if (t < WP) {
-            const int j = t;
-#pragma unroll 1
-            for (int i = 0; i < WP; i++) Xi[i * PP + j] = (i == j) ? T(1) : T(0);
-            if (j < w)
-#pragma unroll 1
-                for (int i = j + 1; i < w; i++) {
-                    T x = T(0);
-#pragma unroll 1
-                    for (int k = j; k < i; k++) x -= P[i * PP + k] * Xi[k * PP + j];
-                    Xi[i * PP + j] = x;
-                }
-        }
-        __syncthreads();
-        // thread (q, c) of a pass moves rows [CH q, CH q + CH) of the list in
-        // column c: all of a pass's reads, a barrier, then its writes (passes
-        // touch disjoint columns)
-        constexpr int CH = 16;
-        const int nmoved = lst[4 * WP];
-        const int nq = (nmoved + CH - 1) / CH;
-        const int no = nc - w;
-        if (nq > 0) {
-            const int cpp = NT / nq, q = t / cpp;
-            for (int c0 = 0; c0 < no; c0 += cpp) {
-                const int cc = c0 + t - q * cpp;
-                const bool act = q < nq && cc < no;
-                const int col = cc < j0 ? cc : cc + w;
-                T v[CH];
-#pragma unroll
-                for (int k = 0; k < CH; k++)
-                    if (act && CH * q + k < nmoved) v[k] = W[(long)lst[2 * WP + CH * q + k] * ldw + col];
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < CH; k++)
-                    if (act && CH * q + k < nmoved) W[(long)lst[CH * q + k] * ldw + col] = v[k];
             }
         }
         __syncthreads();
+        for (int k = 0; k + 1 < w; k++) {
+            const int cnt = (w - k - 1) * (k + 1);
+            for (int idx = t; idx < cnt; idx += NT) {
+                const int i = k + 1 + idx / (k + 1), j = idx % (k + 1);
+                Xi[i * PP + j] -= P[i * PP + k] * Xi[k * PP + j];
+            }
+            __syncthreads();
+        }
         // U12 = L11^-1 A12 on the matrix cores
         const int N2 = nc - j0 - w;
         batch_trsm<T, NT, WP>(W, ldw, j0, w, N2, Xi);
@@ -519,7 +478,7 @@ __device__ __forceinline__ double batch_resid(const double *A0, long lda, int n,
 
 template <typename T, int NT>
 __device__ __forceinline__ void batch_one(const BatchArgs<T> &a, const BatchSys &s, unsigned char *smem, double *lt,
-                                          int *piv, int *perm, double *red_v, int *red_i, int *map, int *lst)
+                                          int *piv, int *perm, double *red_v, int *red_i)
 {
     const int t = threadIdx.x;
     const int n = s.n, nr = s.nrhs, nc = n + nr, ldw = s.ldw;
@@ -531,8 +490,8 @@ __device__ __forceinline__ void batch_one(const BatchArgs<T> &a, const BatchSys 
     }
     for (int i = t; i < n; i += NT) perm[i] = i;
     __syncthreads();
-    if (s.wp == 16) batch_factor<T, NT, 16>(W, ldw, n, nc, (T *)smem, (T *)lt, piv, perm, red_v, red_i, map, lst);
-    else batch_factor<T, NT, 32>(W, ldw, n, nc, (T *)smem, (T *)lt, piv, perm, red_v, red_i, map, lst);
+    if (s.wp == 16) batch_factor<T, NT, 16>(W, ldw, n, nc, (T *)smem, (T *)lt, piv, perm, red_v, red_i);
+    else batch_factor<T, NT, 32>(W, ldw, n, nc, (T *)smem, (T *)lt, piv, perm, red_v, red_i);
 
     double *xv = (double *)smem, *rv = xv + a.vstride, *cv = rv + a.vstride, *blk = cv + a.vstride;
     for (int idx = t; idx < n * nr; idx += NT) {
@@ -577,8 +536,6 @@ __global__ __launch_bounds__(NT) void batch_lu_kernel(BatchArgs<T> a)
     __shared__ double s_lt[32 * 33];   // inverse of the current panel's L11
     __shared__ double red_v[(NT / 64) * (3 * BRHSMAX + 1)];
     __shared__ int red_i[NT / 64];
-    __shared__ int s_map[BNMAX];                // the panel's row exchanges, replayed
-    __shared__ int s_lst[4 * 32 + 1];           // moved rows, their sources, the count
     __shared__ int s_q;
     for (;;) {
         if (threadIdx.x == 0) s_q = (int)atomicAdd(a.queue, 1u);
@@ -586,7 +543,7 @@ __global__ __launch_bounds__(NT) void batch_lu_kernel(BatchArgs<T> a)
         const int q = s_q;
         __syncthreads();
         if (q >= a.nsys) break;   // every wave of the workgroup reads the same q
-        batch_one<T, NT>(a, a.sys[q], smem, s_lt, s_piv, s_perm, red_v, red_i, s_map, s_lst);
+        batch_one<T, NT>(a, a.sys[q], smem, s_lt, s_piv, s_perm, red_v, red_i);
     }
 }
 
