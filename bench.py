@@ -402,6 +402,8 @@ def main() -> int:
                          "step; library default if omitted)")
     ap.add_argument("--wide16", type=int, default=0, choices=(0, 1),
                     help="16-bit trailing updates in 256 x 256 tiles (1) or 128 x 128 (0, default)")
+    ap.add_argument("--backsub128", type=int, default=0, choices=(0, 1),
+                    help="back substitutions in 128-row blocks (1) or 64 (0, default)")
     ap.add_argument("--strict-sync", type=int, default=0, choices=(0, 1),
                     help="leaf and back substitution publish with agent-scope release/acquire")
     ap.add_argument("--gemm-probe", type=int, nargs=3, metavar=("M", "N", "K"), default=None,
@@ -450,6 +452,7 @@ def main() -> int:
     ctx.set_option(L.OPT_TRAIL_KERNEL, args.trail_kernel)
     ctx.set_option(L.OPT_STRICT_SYNC, args.strict_sync)
     ctx.set_option(L.OPT_WIDE16, args.wide16)
+    ctx.set_option(L.OPT_BACKSUB128, args.backsub128)
     if args.subpanel is not None:
         ctx.set_option(L.OPT_SUBPANEL, args.subpanel)
     nbytes = 8 * n * lda
@@ -554,7 +557,8 @@ def main() -> int:
                                f"pivot={args.pivot}, 1 RHS, seed {args.seed}",
                    "n": n, "nb": nb, "pivot": args.pivot, "lookahead": args.lookahead, "leaf_w": args.leaf_w,
                    "subpanel": args.subpanel if args.subpanel is not None else 0,
-                   "trail_kernel": args.trail_kernel, "strict_sync": args.strict_sync, "wide16": args.wide16},
+                   "trail_kernel": args.trail_kernel, "strict_sync": args.strict_sync, "wide16": args.wide16,
+                   "backsub128": args.backsub128},
         "tflops_2n3_3_plus_2n2": (flops + 2.0 * n * n) / mean / 1e12,
         "frac_of_fp64_peak": flops / mean / 1e12 / FP64_MFMA_PEAK_TFLOPS,
         "hpl2_residual": float(res[6]),

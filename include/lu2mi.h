@@ -67,6 +67,10 @@ extern "C" {
 #define LU2MI_OPT_WIDE16 9        /* 1: 16-bit trailing updates in 256 x 256 tiles (128 x 64 per
                                      wave) where they give every CU a workgroup; 0 (default):
                                      128 x 128 tiles (results identical either way) */
+#define LU2MI_OPT_BACKSUB128 10   /* 1: back substitutions (the solve and the refinement's
+                                     re-solves) in 128-row blocks, half as many cross-workgroup
+                                     hand-overs; 0 (default): 64-row blocks.  The two sum each
+                                     row's terms in different orders */
 
 typedef struct lu2mi_ctx lu2mi_ctx;
 

@@ -42,6 +42,7 @@ OPT_TRAIL_KERNEL = 6
 OPT_DGEMM_TRAILING = 7
 OPT_STRICT_SYNC = 8
 OPT_WIDE16 = 9
+OPT_BACKSUB128 = 10
 BATCH_EAGER = 1
 BATCH_NMAX = 512
 BATCH_RHSMAX = 8

@@ -63,6 +63,7 @@ extern "C" int lu2mi_init(int device, lu2mi_ctx **out)
     ctx->trail_kernel = 1;
     ctx->dgemm_as_trailing = 0;
     ctx->wide16 = 0;
+    ctx->backsub128 = 0;
     ctx->sub_w = 0;
     LU2MI_CHECK(hipMalloc(&ctx->d_bar, 256));
     LU2MI_CHECK(hipMemset(ctx->d_bar, 0, 256));
@@ -167,6 +168,9 @@ extern "C" int lu2mi_set_option(lu2mi_ctx *ctx, int opt, int value)
         return LU2MI_OK;
     case LU2MI_OPT_WIDE16:
         ctx->wide16 = value ? 1 : 0;
+        return LU2MI_OK;
+    case LU2MI_OPT_BACKSUB128:
+        ctx->backsub128 = value ? 1 : 0;
         return LU2MI_OK;
     case LU2MI_OPT_TIMING:
         ctx->timing_on = value ? 1 : 0;

@@ -86,6 +86,7 @@ struct lu2mi_ctx {
     int dgemm_as_trailing; // lu2mi_dgemm_dev runs its product as a trailing update
     int strict_sync;       // leaf and back substitution publish with release/acquire
     int wide16;            // 16-bit updates in 256 x 256 tiles where they fill the device
+    int backsub128;        // back substitution in 128-row blocks (64 otherwise)
     hipStream_t stream2;
     hipEvent_t ev_next, ev_panel;
     KernelTimer tcat[LU2MI_T_COUNT];   // per subsystem (LU2MI_T_*)

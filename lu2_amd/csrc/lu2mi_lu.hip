@@ -847,46 +847,52 @@ __device__ __forceinline__ float lane_bcast(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-template <typename TF, typename TX>
-__global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, long lda, int n, TX *x,
-                                                      unsigned long long *front, int *err, int strict)
+template <typename TF, typename TX, int BS>
+__global__ __launch_bounds__(BS * 4) void backsub_kernel(const TF *__restrict__ A, long lda, int n, TX *x,
+                                                         unsigned long long *front, int *err, int strict)
 {
-    __shared__ TF Ud[BSB][BSB + 1];
-    __shared__ TX part[BSB];
+    // BS-row blocks, BS/16 waves of 16 rows each; the columns of a solved
+    // block are taken 64 at a time (lane = column), in BS/64 halves
+    constexpr int NT = BS * 4, NH = BS / 64;
+    __shared__ TF Ud[BS][BS + 1];
+    __shared__ TX part[BS];
     __shared__ int s_front;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int nblk = (n + BSB - 1) / BSB;
+    const int nblk = (n + BS - 1) / BS;
     const int G = gridDim.x;
     const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(x, 0, (int)(sizeof(TX) * n), 0x00020000);
     for (int idx = blockIdx.x; idx < nblk; idx += G) {
         const int b = nblk - 1 - idx;
-        const int r0 = b * BSB, r1 = min(n, r0 + BSB), nrb = r1 - r0;
-        for (int e = t; e < BSB * BSB; e += 256) {
-            const int rr = e / BSB, kk = e % BSB;
+        const int r0 = b * BS, r1 = min(n, r0 + BS), nrb = r1 - r0;
+        for (int e = t; e < BS * BS; e += NT) {
+            const int rr = e / BS, kk = e % BS;
             Ud[rr][kk] = (rr < nrb && kk < nrb) ? A[(long)(r0 + rr) * lda + r0 + kk] : TF(0);
         }
         // wave w accumulates rows r0 + 16w .. r0 + 16w + 15, lane = column in the solved block
         TX acc[16];
-        TF pa[16];
+        TF pa[NH][16];
 #pragma unroll
         for (int i = 0; i < 16; i++) acc[i] = TX(0);
-        // A tile of solved block d for this thread: rows r0 + 16w + i, column c0(d) + lane
-        auto fetch = [&](int d, TF *v) {
-            const int col = (nblk - 1 - d) * BSB + lane;
+        // the 64 columns h of solved block d for this thread: rows r0 + 16w + i,
+        // column c0(d) + 64 h + lane
+        auto fetch = [&](int d, int h, TF *v) {
+            const int col = (nblk - 1 - d) * BS + 64 * h + lane;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const int r = r0 + w * 16 + i;
                 v[i] = (r < r1 && col < n) ? A[(long)r * lda + col] : TF(0);
             }
         };
-        auto accumulate = [&](int d, const TF *v) {
-            const int col = (nblk - 1 - d) * BSB + lane;
+        auto accumulate = [&](int d, int h, const TF *v) {
+            const int col = (nblk - 1 - d) * BS + 64 * h + lane;
             const TX xv = (col < n) ? ld_buf(xr, (int)(sizeof(TX) * col), TX(0)) : TX(0);
 #pragma unroll
             for (int i = 0; i < 16; i++) acc[i] += (TX)v[i] * xv;
         };
         int done = 0;
-        if (idx > 0) fetch(0, pa);
+        if (idx > 0)
+#pragma unroll
+            for (int h = 0; h < NH; h++) fetch(0, h, pa[h]);
         while (done < idx) {
             if (t == 0) {
                 unsigned long spins = 0;
@@ -907,22 +913,29 @@ __global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, 
             __syncthreads();
             const int F = min(s_front, idx);
             acquire_seq(strict);
-            accumulate(done, pa);            // the prefetched tile
+#pragma unroll
+            for (int h = 0; h < NH; h++) accumulate(done, h, pa[h]);   // the prefetched tile
             int d = done + 1;
-            for (; d + 1 < F; d += 2) {      // blocks already solved: two tiles in flight
-                TF v0[16], v1[16];
-                fetch(d, v0);
-                fetch(d + 1, v1);
-                accumulate(d, v0);
-                accumulate(d + 1, v1);
+            if (NH == 1) {
+                for (; d + 1 < F; d += 2) {      // blocks already solved: two tiles in flight
+                    TF v0[16], v1[16];
+                    fetch(d, 0, v0);
+                    fetch(d + 1, 0, v1);
+                    accumulate(d, 0, v0);
+                    accumulate(d + 1, 0, v1);
+                }
             }
-            if (d < F) {
-                TF v0[16];
-                fetch(d, v0);
-                accumulate(d, v0);
+            for (; d < F; d++) {
+                TF v0[16], v1[16];
+                fetch(d, 0, v0);
+                if (NH > 1) fetch(d, NH - 1, v1);
+                accumulate(d, 0, v0);
+                if (NH > 1) accumulate(d, NH - 1, v1);
             }
             done = F;
-            if (done < idx) fetch(done, pa);   // ahead of the next wait
+            if (done < idx)
+#pragma unroll
+                for (int h = 0; h < NH; h++) fetch(done, h, pa[h]);   // ahead of the next wait
             __syncthreads();
         }
 #pragma unroll
@@ -934,13 +947,22 @@ __global__ __launch_bounds__(256) void backsub_kernel(const TF *__restrict__ A, 
         }
         __syncthreads();
         if (t < 64) {
-            // right-looking: x[rr] is final when step rr begins; rows above subtract U[t][rr] x[rr]
-            TX xt = (t < nrb) ? x[r0 + t] - part[t] : TX(0);
+            // right-looking: x[rr] is final when step rr begins; rows above
+            // subtract U[row][rr] x[rr]; lane t holds rows t + 64 h
+            TX xt[NH];
+#pragma unroll
+            for (int h = 0; h < NH; h++) xt[h] = (t + 64 * h < nrb) ? x[r0 + t + 64 * h] - part[t + 64 * h] : TX(0);
             for (int rr = nrb - 1; rr > 0; rr--) {
-                const TX xk = lane_bcast(xt, rr);
-                if (t < rr) xt -= (TX)Ud[t][rr] * xk;
+                TX xk;
+                if (NH == 1 || rr < 64) xk = lane_bcast(xt[0], rr & 63);
+                else xk = lane_bcast(xt[NH - 1], rr & 63);
+#pragma unroll
+                for (int h = 0; h < NH; h++)
+                    if (t + 64 * h < rr) xt[h] -= (TX)Ud[t + 64 * h][rr] * xk;
             }
-            if (t < nrb) st_dev(x + r0 + t, xt);
+#pragma unroll
+            for (int h = 0; h < NH; h++)
+                if (t + 64 * h < nrb) st_dev(x + r0 + t + 64 * h, xt[h]);
             // the block's x all come from this one wave, whose s_waitcnt
             // vmcnt(0) in publish_seq covers every lane's store
             if (!strict && t == 0) publish_seq(front, (unsigned long long)(idx + 1), 0);
@@ -1118,21 +1140,30 @@ static int gemm_call(lu2mi_ctx *ctx, int M, int N, int K, double alpha, const T 
 }
 
 // x <- U^-1 x with the unit-upper U stored in A (one persistent launch)
-template <typename TF, typename TX>
-static int backsub(lu2mi_ctx *ctx, const TF *A, long lda, int n, TX *x)
+template <typename TF, typename TX, int BS>
+static int backsub_bs(lu2mi_ctx *ctx, const TF *A, long lda, int n, TX *x)
 {
-    const int nblk = (n + BSB - 1) / BSB;
+    const int nblk = (n + BS - 1) / BS;
     int per_cu = 0;
-    LU2MI_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)backsub_kernel<TF, TX>, 256, 0));
-    // every workgroup the device keeps co-resident (2 per CU for these
-    // kernels): more workgroups mean shorter backlogs of solved tiles each
+    LU2MI_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)backsub_kernel<TF, TX, BS>, 4 * BS, 0));
+    // every workgroup the device keeps co-resident: more workgroups mean
+    // shorter backlogs of solved tiles each
     int G = per_cu * ctx->num_cu < nblk ? per_cu * ctx->num_cu : nblk;
     if (G < 1) G = 1;
     LU2MI_CHECK(hipMemsetAsync(ctx->d_bar, 0, sizeof(unsigned long long), ctx->stream));
-    hipLaunchKernelGGL((backsub_kernel<TF, TX>), dim3(G), dim3(256), 0, ctx->stream, A, lda, n, x, ctx->d_bar,
+    hipLaunchKernelGGL((backsub_kernel<TF, TX, BS>), dim3(G), dim3(4 * BS), 0, ctx->stream, A, lda, n, x, ctx->d_bar,
                        ctx->d_err, ctx->strict_sync);
     LU2MI_CHECK(hipGetLastError());
     return LU2MI_OK;
+}
+
+// 64-row blocks, or 128 (LU2MI_OPT_BACKSUB128): half as many frontier steps,
+// each a cross-workgroup hand-over, for a longer diagonal solve per step
+template <typename TF, typename TX>
+static int backsub(lu2mi_ctx *ctx, const TF *A, long lda, int n, TX *x)
+{
+    if (ctx->backsub128) return backsub_bs<TF, TX, 128>(ctx, A, lda, n, x);
+    return backsub_bs<TF, TX, BSB>(ctx, A, lda, n, x);
 }
 
 template <typename T>

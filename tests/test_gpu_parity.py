@@ -417,3 +417,68 @@ def test_subpanel_option_rejects(ctx):
     for bad in (8, 48, 512, -1):
         with pytest.raises(L.Lu2miError):
             ctx.set_option(L.OPT_SUBPANEL, bad)
+
+
+@pytest.mark.parametrize("n,nb,mode", [(4096, 256, L.PIV_GEPP), (3040, 32, L.PIV_GEPP), (2048, 512, L.PIV_COMPAT)])
+def test_backsub128_solution(ctx, oracle, n, nb, mode):
+    """Back substitution in 128-row blocks (LU2MI_OPT_BACKSUB128): the
+    factors and pivots are the 64-row run's bits (only the solve differs);
+    x is the same unit-upper solve summed in another order, so it agrees
+    with the 64-row x to the solve's rounding, and with the oracle's x as the
+    64-row solve does.  n = 3040 leaves a ragged last 128-row block."""
+    lda = n + 16
+    outs = []
+    for b128 in (0, 1):
+        dA = ctx.alloc(8 * n * lda)
+        dx = ctx.alloc(8 * n)
+        dp = ctx.alloc(4 * n)
+        ctx.randomsetmat(n, 5, dA, lda)
+        ctx.set_option(L.OPT_PIVMODE, mode)
+        ctx.set_option(L.OPT_BACKSUB128, b128)
+        try:
+            ctx.lumcolumn_dev(n, dA, lda, nb, dp, dx)
+        finally:
+            ctx.set_option(L.OPT_BACKSUB128, 0)
+        a = dA.download(np.float64, n * lda).reshape(n, lda)
+        outs.append((a[:, :n].copy(), dx.download(np.float64, n), dp.download(np.int32, n), a))
+        for d in (dA, dx, dp):
+            d.free()
+    (f0, x0, p0, _), (f1, x1, p1, a1) = outs
+    assert np.array_equal(f0, f1) and np.array_equal(p0, p1)
+    if mode == L.PIV_GEPP:
+        # two backward-stable triangular solves of a well-conditioned U
+        assert np.abs(x1 - x0).max() <= 1e-12 * np.abs(x0).max()
+        xo, po, _ = oracle.lumcolumn(oracle.randomsetmat(n, 5), nb, oracle.PIV_GEPP)
+        assert np.array_equal(p1, po)
+        assert np.abs(x1 - xo).max() <= 1e-10 * np.abs(xo).max()
+    # the 128-row solve against the factors themselves (both pivot rules):
+    # U x1 = y with U the unit upper triangle and y column n after the
+    # elimination, to the backward error of a triangular solve (a wrong
+    # index or a dropped term shows at O(1))
+    U = np.triu(f1, 1) + np.eye(n)
+    y = a1[:, n]
+    r = np.abs(U @ x1 - y).max() / (np.abs(U).sum(1).max() * np.abs(x1).max() * n * 2.0 ** -53)
+    assert r < 16.0
+
+
+def test_backsub128_refinement(ctx, oracle):
+    """The mixed solve's re-solves with 128-row blocks (FP32 factors, FP64
+    vector): same target met, x within 1e-10 of the oracle's."""
+    n, nb = 4096, 512
+    lda = n + 16
+    a0 = oracle.randomsetmat(n, 3)
+    dA = ctx.alloc(8 * n * lda)
+    dx = ctx.alloc(8 * n)
+    try:
+        dA.upload(a0)
+        ctx.set_option(L.OPT_PIVMODE, L.PIV_GEPP)
+        ctx.set_option(L.OPT_BACKSUB128, 1)
+        info = ctx.solve_mixed(n, dA, lda, nb, dx, L.PREC_FP32, maxiter=30, target=1.0)
+        x = dx.download(np.float64, n)
+    finally:
+        ctx.set_option(L.OPT_BACKSUB128, 0)
+        dA.free()
+        dx.free()
+    assert info["resid_hpl2"] <= 1.0
+    xo, _, _ = oracle.lumcolumn(a0, nb, oracle.PIV_GEPP)
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
