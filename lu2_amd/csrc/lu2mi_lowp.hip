@@ -111,12 +111,6 @@ __global__ __launch_bounds__(256) void scale_cols16_kernel(const float *__restri
 
 // C[M x N] -= 2^-(sa[tm]+sb[tn]) * A16 B16t^T; A16 and B16t padded to whole
 // 128-row tiles.  256 threads, 2x2 waves of 64x64, 16x16x32 matrix-core
-// blocks; K staged 32 at a time through LDS with a register double buffer.
-// Lane l supplies A[l%16][8(l/16)+e] and B[8(l/16)+e][l%16]: the same K
-// assignment in both operands, so the product is the GEMM's whatever order
-// the instruction sums in; the result lands at row 4(l/16)+r, column l%16.
-// C[M x N] -= 2^-(sa[tm]+sb[tn]) * A16 B16t^T; A16 and B16t padded to whole
-// 128-row tiles.  256 threads, 2x2 waves of 64x64, 16x16x32 matrix-core
 // blocks.  K is staged 64 at a time through LDS; the next stage's global
 // loads are in flight in registers while the current stage computes.  One
 // operand plane keeps two LDS buffers (one barrier per stage); the split
