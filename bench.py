@@ -1,6 +1,6 @@
 """Benchmark: lu2 LU + solve on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 65536] [--nb 2048]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 65536] [--nb 1024]
                     [--pivot gepp|compat] [--no-cpu] [--no-mixed] [--no-lowp]
                     [--no-batch]
 
@@ -52,6 +52,10 @@ FP64_MFMA_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix, published
 TRAIL_KERNEL_NAME = {1: "gemm2_kernel<1> (trailing update A22 -= L21*U12, operands loaded straight into LDS)",
                      0: "gemm_kernel<double, true, 1, 1> (trailing update A22 -= L21*U12, register-staged)"}
 HBM_PEAK_GBS = 8000.0
+# HBM bytes of one M=N=32768 trailing update per (tile, K), measured with
+# rocprofv3 FETCH_SIZE (x 2.000) + WRITE_SIZE (x 1.000) on bench.py --gemm-probe
+TRAIL_TRAFFIC = {(1, 2048): (68.70e9, "trailing_dgemm_m32768_k2048_pmc.md"),
+                 (1, 1024): (42.91e9, "trailing_dgemm_m32768_k1024_pmc.md")}
 
 
 def log(msg: str) -> None:
@@ -378,7 +382,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=65536)
-    ap.add_argument("--nb", type=int, default=2048)
+    ap.add_argument("--nb", type=int, default=1024)
     ap.add_argument("--pivot", choices=["gepp", "compat"], default="gepp")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -569,13 +573,13 @@ def main() -> int:
             "peak": FP64_MFMA_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved / FP64_MFMA_PEAK_TFLOPS) if achieved else None,
-            # bytes of one trailing update of the factorization's M=N=32768, K=2048 shape,
-            # from FETCH_SIZE and WRITE_SIZE counter runs scaled on a 1 GiB copy
-            # (profiles/trailing_dgemm_m32768_k2048_pmc.md); the algorithmic bytes of
-            # that launch are 18.25e9
-            "traffic": 68.70e9,
-            "traffic_launch": "M=N=32768 K=2048",
-            "traffic_algorithmic": 18.25e9,
+            # bytes of one trailing update of the factorization's M=N=32768, K=nb shape,
+            # from FETCH_SIZE and WRITE_SIZE counter runs of --gemm-probe scaled on
+            # its 1 GiB copy (profiles/trailing_dgemm_m32768_k{nb}_pmc.md); None for
+            # a block size without such a run
+            "traffic": TRAIL_TRAFFIC.get((args.trail_kernel, nb), (None, None))[0],
+            "traffic_launch": f"M=N=32768 K={nb}",
+            "traffic_algorithmic": 8.0 * (2 * 32768 * 32768 + 2 * 32768 * nb),
             "launches_per_step": gemm_launch // max(1, args.steps),
             "avg_launch_ms": gemm_ms / max(1, gemm_launch),
         },

@@ -8,7 +8,7 @@
   the leaf holds four rows per thread.
 * The device residual (lu2.c:45-60 Emax, lu2_mpi.c:2170-2229 resid1..3 and
   the HPL-2 form) against the oracle's residuals on the same x.
-* n = 65536 (FP64 and both mixed-precision forms) and n = 98304 (FP16 and
+* n = 65536 (FP64 at nb = 1024 and both mixed-precision forms) and n = 98304 (FP16 and
   BF16 updates on the diagonally dominant matrix) through properties that do
   not depend on size: HPL-2 scaled residual, a valid pivot sequence, the
   mixed solutions against the FP64 one.
@@ -151,13 +151,15 @@ def test_residual_dev_keeps_nan(ctx, oracle):
 
 
 # ------------------------------------------------- full-size properties
-N3, NB3 = 65536, 2048
+N3, NB3 = 65536, 1024
+# block size of each mixed form at n=65536 (as bench.py runs them)
+NB_MIXED = {L.PREC_FP16X2: 2048, L.PREC_FP32: 1024}
 
 
 @pytest.fixture(scope="module")
 def fp64_n65536(ctx):
     """The metric's configuration: FP64 LU + solve of lu2's matrix, n=65536,
-    nb=2048, GEPP; the pristine matrix stays resident for the mixed solves."""
+    nb=1024, GEPP; the pristine matrix stays resident for the mixed solves."""
     n, lda = N3, N3 + 16
     dA0 = ctx.alloc(8 * n * lda)
     dA = ctx.alloc(8 * n * lda)
@@ -188,7 +190,7 @@ def test_mixed_n65536(ctx, fp64_n65536, prec):
     dx = ctx.alloc(8 * n)
     try:
         ctx.set_option(L.OPT_PIVMODE, L.PIV_GEPP)
-        info = ctx.solve_mixed(n, fp64_n65536["dA0"], lda, NB3, dx, prec, maxiter=30, target=1.0)
+        info = ctx.solve_mixed(n, fp64_n65536["dA0"], lda, NB_MIXED[prec], dx, prec, maxiter=30, target=1.0)
         x = dx.download(np.float64, n)
         res = ctx.residual_dev(n, fp64_n65536["dA0"], lda, dx)
     finally:
