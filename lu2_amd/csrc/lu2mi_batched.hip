@@ -95,52 +95,68 @@ __device__ __forceinline__ void batch_update(T *W, int ldw, int j0, int w, int M
     constexpr int PP = WP + 1, NW = NT / 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int l16 = lane & 15, lk = lane >> 4;
-    const int tm_n = (M2 + 31) >> 5, tn_n = (N2 + 31) >> 5;
+    const int tm_n = (M2 + 31) >> 5, tn_n = (N2 + 31) >> 5, ntile = tm_n * tn_n;
     T *C0 = W + (long)(j0 + w) * ldw + j0 + w;
     const T *B0 = W + (long)j0 * ldw + j0 + w;
-    for (int tile = wv; tile < tm_n * tn_n; tile += NW) {
-        const int tm = tile / tn_n, tn = tile - tm * tn_n;
-        const int r0 = tm * 32, c0 = tn * 32;
-        typename F::acc_t acc[2][2];
+    // two tiles per wave at a time (tile and tile + NW), so that the C and B
+    // loads of both are in flight together: each tile alone is one load
+    // latency, a few matrix-core steps and a store
+    for (int tile = wv; tile < ntile; tile += 2 * NW) {
+        typename F::acc_t acc[2][2][2];   // [pair][bi][bj]
+        int r0[2], c0[2];
+        bool on[2];
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++)
+        for (int u = 0; u < 2; u++) {
+            const int tl = tile + u * NW;
+            on[u] = tl < ntile;
+            const int tm = on[u] ? tl / tn_n : 0, tn = on[u] ? tl - tm * tn_n : 0;
+            r0[u] = tm * 32;
+            c0[u] = tn * 32;
 #pragma unroll
-            for (int bj = 0; bj < 2; bj++)
+            for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    const int row = r0 + bi * 16 + F::out_row(lane, rr), col = c0 + bj * 16 + l16;
-                    acc[bi][bj][rr] = (row < M2 && col < N2) ? C0[(long)row * ldw + col] : T(0);
-                }
+                for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const int row = r0[u] + bi * 16 + F::out_row(lane, rr), col = c0[u] + bj * 16 + l16;
+                        acc[u][bi][bj][rr] = (on[u] && row < M2 && col < N2) ? C0[(long)row * ldw + col] : T(0);
+                    }
+        }
 #pragma unroll
         for (int k0 = 0; k0 < WP; k0 += 4) {
             if (k0 < w) {
                 const int k = k0 + lk;
-                T af[2], bf[2];
 #pragma unroll
-                for (int bi = 0; bi < 2; bi++) {
-                    const int row = r0 + bi * 16 + l16;
-                    af[bi] = (k < w && row < M2) ? -P[(w + row) * PP + k] : T(0);
+                for (int u = 0; u < 2; u++) {
+                    T af[2], bf[2];
+#pragma unroll
+                    for (int bi = 0; bi < 2; bi++) {
+                        const int row = r0[u] + bi * 16 + l16;
+                        af[bi] = (on[u] && k < w && row < M2) ? -P[(w + row) * PP + k] : T(0);
+                    }
+#pragma unroll
+                    for (int bj = 0; bj < 2; bj++) {
+                        const int col = c0[u] + bj * 16 + l16;
+                        bf[bj] = (on[u] && k < w && col < N2) ? B0[(long)k * ldw + col] : T(0);
+                    }
+#pragma unroll
+                    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+                        for (int bj = 0; bj < 2; bj++) acc[u][bi][bj] = F::step(af[bi], bf[bj], acc[u][bi][bj]);
                 }
-#pragma unroll
-                for (int bj = 0; bj < 2; bj++) {
-                    const int col = c0 + bj * 16 + l16;
-                    bf[bj] = (k < w && col < N2) ? B0[(long)k * ldw + col] : T(0);
-                }
-#pragma unroll
-                for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-                    for (int bj = 0; bj < 2; bj++) acc[bi][bj] = F::step(af[bi], bf[bj], acc[bi][bj]);
             }
         }
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++)
+        for (int u = 0; u < 2; u++)
 #pragma unroll
-            for (int bj = 0; bj < 2; bj++)
+            for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    const int row = r0 + bi * 16 + F::out_row(lane, rr), col = c0 + bj * 16 + l16;
-                    if (row < M2 && col < N2) C0[(long)row * ldw + col] = acc[bi][bj][rr];
-                }
+                for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const int row = r0[u] + bi * 16 + F::out_row(lane, rr), col = c0[u] + bj * 16 + l16;
+                        if (on[u] && row < M2 && col < N2) C0[(long)row * ldw + col] = acc[u][bi][bj][rr];
+                    }
     }
 }
 
@@ -402,33 +418,43 @@ __device__ __forceinline__ double batch_resid(const double *A0, long lda, int n,
 {
     constexpr int NW = NT / 64;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    // a wave takes 8 rows at a time, 8 lanes per row (lane sl of row sub
+    // reads elements sl, sl + 8, ...: 64-byte pieces of 8 rows per load), so
+    // a row's sums need 3 shuffle steps instead of a 6-step wave reduction
+    const int sub = lane >> 3, sl = lane & 7;
     double amax = 0.0;
-    for (int i = wv; i < n; i += NW) {
+    for (int i0 = 8 * wv; i0 < n; i0 += 8 * NW) {
+        const int i = i0 + sub;
         const double *row = A0 + (long)i * lda;
         double s[BRHSMAX];
 #pragma unroll
         for (int c = 0; c < BRHSMAX; c++) s[c] = 0.0;
         double asum = 0.0;
-        for (int j = lane; j < n; j += 64) {
-            const double a = row[j];
-            asum += fabs(a);
+        if (i < n)
+            for (int j = sl; j < n; j += 8) {
+                const double a = row[j];
+                asum += fabs(a);
+#pragma unroll
+                for (int c = 0; c < BRHSMAX; c++)
+                    if (c < nr) s[c] = fma(a, x[j * nr + c], s[c]);
+            }
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            asum += __shfl_xor(asum, o, 64);
 #pragma unroll
             for (int c = 0; c < BRHSMAX; c++)
-                if (c < nr) s[c] = fma(a, x[j * nr + c], s[c]);
+                if (c < nr) s[c] += __shfl_xor(s[c], o, 64);
         }
-        asum = wave_sum(asum);
-#pragma unroll
-        for (int c = 0; c < BRHSMAX; c++)
-            if (c < nr) s[c] = wave_sum(s[c]);
         amax = nanmax(amax, asum);
-        if (lane < nr) {
+        if (i < n && sl < nr) {
             double sc = s[0];
 #pragma unroll
             for (int c = 1; c < BRHSMAX; c++)
-                if (lane == c) sc = s[c];
-            r[i * nr + lane] = row[n + lane] - sc;
+                if (sl == c) sc = s[c];
+            r[i * nr + sl] = row[n + sl] - sc;
         }
     }
+    amax = wave_nanmax(amax);   // lanes of different rows hold different maxima
     // per column maxima of |r|, |x|, |b|; the row-sum maximum
     double mr[BRHSMAX], mx[BRHSMAX], mb[BRHSMAX];
 #pragma unroll
@@ -484,9 +510,20 @@ __device__ __forceinline__ void batch_one(const BatchArgs<T> &a, const BatchSys 
     const int n = s.n, nr = s.nrhs, nc = n + nr, ldw = s.ldw;
     const double *A0 = a.A + s.a_off;
     T *W = a.Wk + s.w_off;
-    for (long idx = t; idx < (long)n * nc; idx += NT) {
-        const int i = (int)(idx / nc), j = (int)(idx - (long)i * nc);
-        W[(long)i * ldw + j] = (T)A0[(long)i * s.lda + j];
+    // a wave per row, lanes along it; a row's loads are all issued before its
+    // stores (nc <= BNMAX + BRHSMAX = 9 x 64)
+    for (int i = t >> 6; i < n; i += NT / 64) {
+        T v[(BNMAX + BRHSMAX + 63) / 64];
+#pragma unroll
+        for (int k = 0; k < (BNMAX + BRHSMAX + 63) / 64; k++) {
+            const int j = (t & 63) + 64 * k;
+            v[k] = (j < nc) ? (T)A0[(long)i * s.lda + j] : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < (BNMAX + BRHSMAX + 63) / 64; k++) {
+            const int j = (t & 63) + 64 * k;
+            if (j < nc) W[(long)i * ldw + j] = v[k];
+        }
     }
     for (int i = t; i < n; i += NT) perm[i] = i;
     __syncthreads();
