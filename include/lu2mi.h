@@ -55,8 +55,9 @@ extern "C" {
                                      loaded straight into LDS (default), 0 = staged through
                                      registers; the two sum each product's K terms in
                                      different orders */
-#define LU2MI_OPT_DGEMM_TRAILING 7 /* 1: lu2mi_dgemm_dev runs its product with the
-                                     trailing-update kernel (to measure that kernel alone);
+#define LU2MI_OPT_DGEMM_TRAILING 7 /* 1: lu2mi_dgemm_dev (and lu2mi_sgemm_dev) run their
+                                     product with the trailing-update kernel (to measure or
+                                     test that kernel alone);
                                      2: the same, with A taken as lower triangular (the U12
                                      product's form: entries above the diagonal are not read) */
 #define LU2MI_OPT_STRICT_SYNC 8   /* 1: the panel leaf and the back substitution publish their

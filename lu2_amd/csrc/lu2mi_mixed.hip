@@ -473,6 +473,7 @@ extern "C" int lu2mi_sgemm_dev(lu2mi_ctx *ctx, int m, int n, int k, float alpha,
     GemmDesc g;
     g.M = m; g.N = n; g.K = k; g.alpha = alpha; g.beta = beta;
     g.A = dA; g.lda = lda; g.B = dB; g.ldb = ldb; g.C = dC; g.ldc = ldc;
+    if (ctx->dgemm_as_trailing == 1) g.tag = ctx->trail_kernel ? 1 : 2;   // the FP32 trailing-update kernel
     LU2MI_TRY(lu2mi_gemm_f32(g, ctx->stream));
     LU2MI_CHECK(hipStreamSynchronize(ctx->stream));
     return LU2MI_OK;
