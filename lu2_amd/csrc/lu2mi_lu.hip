@@ -976,35 +976,71 @@ __global__ __launch_bounds__(BS * 4) void backsub_kernel(const TF *__restrict__ 
 
 // ---------------------------------------------------------------------------
 // FP64 residual: per row |(A x)_i - b_i| and sum_j |a_ij|; per column sum |a_ij|.
+// A wavefront takes RESID_ROWS rows, so each x[c] it loads serves all of them;
+// each row is summed in the same order as by a wavefront of its own.
+constexpr int RESID_ROWS = 4;
 __global__ __launch_bounds__(256) void resid_rows_kernel(const double *A, long lda, int n, const double *x,
                                                          double *rerr, double *rabs)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long r = (long)blockIdx.x * 4 + wave;
-    if (r >= n) return;
-    const double *a = A + r * lda;
-    double s = 0.0, sa = 0.0;
+    const long r0 = ((long)blockIdx.x * 4 + wave) * RESID_ROWS;
+    if (r0 >= n) return;
+    const int nr = (int)min((long)RESID_ROWS, (long)n - r0);
+    const double *a = A + r0 * lda;
+    double s[RESID_ROWS], sa[RESID_ROWS];
+#pragma unroll
+    for (int q = 0; q < RESID_ROWS; q++) s[q] = sa[q] = 0.0;
     for (int c = lane; c < n; c += 64) {
-        s = fma(a[c], x[c], s);
-        sa += fabs(a[c]);
+        const double xc = x[c];
+#pragma unroll
+        for (int q = 0; q < RESID_ROWS; q++) {
+            if (q < nr) {
+                const double e = a[q * lda + c];
+                s[q] = fma(e, xc, s[q]);
+                sa[q] += fabs(e);
+            }
+        }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        s += __shfl_xor(s, off);
-        sa += __shfl_xor(sa, off);
+#pragma unroll
+    for (int q = 0; q < RESID_ROWS; q++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            s[q] += __shfl_xor(s[q], off);
+            sa[q] += __shfl_xor(sa[q], off);
+        }
     }
     if (lane == 0) {
-        rerr[r] = fabs(s - a[n]);
-        rabs[r] = sa;
+        for (int q = 0; q < nr; q++) {
+            rerr[r0 + q] = fabs(s[q] - a[q * lda + n]);
+            rabs[r0 + q] = sa[q];
+        }
     }
 }
 
+// A workgroup takes 64 columns; its 4 waves take the rows r = w (mod 4), each
+// lane four rows per step (four loads in flight), and the 16 partial sums per
+// column are added in a fixed order, so the result does not depend on timing.
 __global__ __launch_bounds__(256) void resid_cols_kernel(const double *A, long lda, int n, double *cabs)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n) return;
-    double s = 0.0;
-    for (long r = 0; r < n; r++) s += fabs(A[r * lda + col]);
-    cabs[col] = s;
+    __shared__ double part[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = blockIdx.x * 64 + lane;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (col < n) {
+        const double *a = A + col;
+        long r = wave;
+        for (; r + 12 < n; r += 16) {
+            s0 += fabs(a[r * lda]);
+            s1 += fabs(a[(r + 4) * lda]);
+            s2 += fabs(a[(r + 8) * lda]);
+            s3 += fabs(a[(r + 12) * lda]);
+        }
+        if (r < n) s0 += fabs(a[r * lda]);
+        if (r + 4 < n) s1 += fabs(a[(r + 4) * lda]);
+        if (r + 8 < n) s2 += fabs(a[(r + 8) * lda]);
+    }
+    part[wave][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (wave == 0 && col < n) cabs[col] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 // out[0]=max rerr, [1]=max rabs, [2]=max cabs, [3]=max|x|, [4]=sum|x|, [5]=max|b|
@@ -1676,8 +1712,9 @@ static int residual_launch(lu2mi_ctx *ctx, int n, const double *dA0, long lda, c
                            double *dout)
 {
     double *rerr = buf, *rabs = buf + n, *cabs = buf + 2 * (size_t)n;
-    hipLaunchKernelGGL(resid_rows_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, dA0, lda, n, dx, rerr, rabs);
-    hipLaunchKernelGGL(resid_cols_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, dA0, lda, n, cabs);
+    hipLaunchKernelGGL(resid_rows_kernel, dim3((n + 4 * RESID_ROWS - 1) / (4 * RESID_ROWS)), dim3(256), 0, ctx->stream,
+                       dA0, lda, n, dx, rerr, rabs);
+    hipLaunchKernelGGL(resid_cols_kernel, dim3((n + 63) / 64), dim3(256), 0, ctx->stream, dA0, lda, n, cabs);
     hipLaunchKernelGGL(resid_final_kernel, dim3(1), dim3(256), 0, ctx->stream, dA0, lda, n, dx, rerr, rabs, cabs,
                        dout);
     LU2MI_CHECK(hipGetLastError());

@@ -99,26 +99,45 @@ __global__ __launch_bounds__(256) void l21_gemv_kernel(const TF *A, long lda, in
 }
 
 // out = b - A v (mode 1, b = column n) or out = A v (mode 0); also |row| sums
-// of A when rowabs is given.  FP64, one wavefront per row.
+// of A when rowabs is given.  FP64; a wavefront takes GEMV_ROWS rows, so each
+// v[c] it loads serves all of them, and each row is summed in the same order
+// as by a wavefront of its own.
+constexpr int GEMV_ROWS = 4;
 __global__ __launch_bounds__(256) void gemv_kernel(const double *A, long lda, int n, const double *v, double *out,
                                                    int mode, double *rowabs)
 {
     const int lane = threadIdx.x & 63;
-    const long j = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (j >= n) return;
-    const double *a = A + j * lda;
-    double s = 0.0, sa = 0.0;
+    const long j0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * GEMV_ROWS;
+    if (j0 >= n) return;
+    const int nr = (int)min((long)GEMV_ROWS, (long)n - j0);
+    const double *a = A + j0 * lda;
+    double s[GEMV_ROWS], sa[GEMV_ROWS];
+#pragma unroll
+    for (int q = 0; q < GEMV_ROWS; q++) s[q] = sa[q] = 0.0;
     for (int c = lane; c < n; c += 64) {
-        s = fma(a[c], v[c], s);
-        if (rowabs) sa += fabs(a[c]);
+        const double vc = v[c];
+#pragma unroll
+        for (int q = 0; q < GEMV_ROWS; q++) {
+            if (q < nr) {
+                const double e = a[q * lda + c];
+                s[q] = fma(e, vc, s[q]);
+                if (rowabs) sa[q] += fabs(e);
+            }
+        }
     }
-    s = wave_sum(s);
-    if (rowabs) sa = wave_sum(sa);
+#pragma unroll
+    for (int q = 0; q < GEMV_ROWS; q++) {
+        s[q] = wave_sum(s[q]);
+        if (rowabs) sa[q] = wave_sum(sa[q]);
+    }
     if (lane == 0) {
-        out[j] = mode ? a[n] - s : s;
-        if (rowabs) rowabs[j] = sa;
+        for (int q = 0; q < nr; q++) {
+            out[j0 + q] = mode ? a[q * lda + n] - s[q] : s[q];
+            if (rowabs) rowabs[j0 + q] = sa[q];
+        }
     }
 }
+static unsigned gemv_blocks(int n) { return (unsigned)((n + 4 * GEMV_ROWS - 1) / (4 * GEMV_ROWS)); }
 
 // per-workgroup partials: part[g*m + q] = sum over the workgroup's range of
 // x*V_q (dots, q < m), or max |x| (m == 0 -> part[g] = max|x|, part[G+g] = sum x^2)
@@ -226,7 +245,7 @@ struct Mixed {
     // r <- b - A x (FP64); returns ||r||_oo and ||x||_oo
     int residual(const double *x, double *r, double *rn, double *xn)
     {
-        hipLaunchKernelGGL(gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s(), A, lda, n, x, r, 1, (double *)nullptr);
+        hipLaunchKernelGGL(gemv_kernel, dim3(gemv_blocks(n)), dim3(256), 0, s(), A, lda, n, x, r, 1, (double *)nullptr);
         LU2MI_CHECK(hipGetLastError());
         double a[2];
         LU2MI_TRY(norms(r, a));
@@ -302,7 +321,7 @@ struct Mixed {
         g[0] = beta;
         int k = 0;
         for (; k < m; k++) {
-            hipLaunchKernelGGL(gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s(), A, lda, n, V(k), w, 0,
+            hipLaunchKernelGGL(gemv_kernel, dim3(gemv_blocks(n)), dim3(256), 0, s(), A, lda, n, V(k), w, 0,
                                (double *)nullptr);
             LU2MI_CHECK(hipGetLastError());
             LU2MI_TRY(lu_solve(w));
@@ -410,7 +429,7 @@ extern "C" int lu2mi_solve_mixed(lu2mi_ctx *ctx, int n, const double *dA, long l
         double *rowabs = M.V(gm + 5), *bcol = M.V(gm + 4);
         hipLaunchKernelGGL(x0_b_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, xf, dA, lda, n, dx, bcol);
         LU2MI_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, dA, lda, n, dx, r, 1, rowabs);
+        hipLaunchKernelGGL(gemv_kernel, dim3(gemv_blocks(n)), dim3(256), 0, ctx->stream, dA, lda, n, dx, r, 1, rowabs);
         LU2MI_CHECK(hipGetLastError());
         double a[2];
         LU2MI_TRY(M.norms(rowabs, a));
