@@ -829,11 +829,16 @@ __global__ void copy_col_kernel(const T *A, long lda, int n, int col, T *x)
 // workgroup accumulates U[block rows][solved block] * x[solved block] for
 // every block below it as soon as the shared frontier (count of solved
 // blocks) shows it solved -- the A tile of the next block it will need is
-// loaded into registers before it waits, so a handoff costs only the x load
-// -- then solves its 64x64 diagonal block right-looking in one wavefront
-// (each finished x broadcast with readlane), publishes x with device-scope
-// stores and advances the frontier.  x holds y on entry.  TF is the factor's
-// element type, TX the vector's (FP64 for refinement).
+// loaded into registers before it waits -- then solves its 64x64 diagonal
+// block right-looking in one wavefront (each finished x broadcast with
+// readlane), publishes x with device-scope stores and advances the frontier.
+// With 64-row blocks the adjacent block (the last one solved before this one)
+// is left to wavefront 0: the other blocks' sums are reduced across lanes
+// while that block is still being solved, its 64x64 tile waits transposed in
+// LDS, and at the hand-over one wavefront applies it (lane = row) and goes on
+// to the diagonal solve with no workgroup barrier in between.  x holds y on
+// entry.  TF is the factor's element type, TX the vector's (FP64 for
+// refinement).
 constexpr int BSB = 64;
 
 __device__ __forceinline__ double lane_bcast(double v, int l)
@@ -854,7 +859,9 @@ __global__ __launch_bounds__(BS * 4) void backsub_kernel(const TF *__restrict__ 
     // BS-row blocks, BS/16 waves of 16 rows each; the columns of a solved
     // block are taken 64 at a time (lane = column), in BS/64 halves
     constexpr int NT = BS * 4, NH = BS / 64;
+    constexpr bool ADJ = (BS == 64);   // the adjacent block applied by wavefront 0
     __shared__ TF Ud[BS][BS + 1];
+    __shared__ TF UaT[ADJ ? BS : 1][BS + 1];   // UaT[k][row]: the tile against the adjacent block
     __shared__ TX part[BS];
     __shared__ int s_front;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -867,6 +874,14 @@ __global__ __launch_bounds__(BS * 4) void backsub_kernel(const TF *__restrict__ 
         for (int e = t; e < BS * BS; e += NT) {
             const int rr = e / BS, kk = e % BS;
             Ud[rr][kk] = (rr < nrb && kk < nrb) ? A[(long)(r0 + rr) * lda + r0 + kk] : TF(0);
+        }
+        if constexpr (ADJ) {
+            // columns [r0 + BS, r0 + 2 BS) of the block's rows (none for the bottom block)
+            const int c0 = r0 + BS, nc = min(BS, n - c0);
+            for (int e = t; e < BS * BS; e += NT) {
+                const int rr = e / BS, kk = e % BS;
+                UaT[kk][rr] = (rr < nrb && kk < nc) ? A[(long)(r0 + rr) * lda + c0 + kk] : TF(0);
+            }
         }
         // wave w accumulates rows r0 + 16w .. r0 + 16w + 15, lane = column in the solved block
         TX acc[16];
@@ -889,11 +904,20 @@ __global__ __launch_bounds__(BS * 4) void backsub_kernel(const TF *__restrict__ 
 #pragma unroll
             for (int i = 0; i < 16; i++) acc[i] += (TX)v[i] * xv;
         };
+        // this block's y, loaded before the waits: only this workgroup writes
+        // these rows of x, and only once it solves the block
+        TX yv[NH];
+        if (t < 64)
+#pragma unroll
+            for (int h = 0; h < NH; h++) yv[h] = (t + 64 * h < nrb) ? x[r0 + t + 64 * h] : TX(0);
+        // solved blocks summed by all waves (lane = column): all below this
+        // one, or all but the adjacent one
+        const int need = ADJ ? idx - 1 : idx;
         int done = 0;
-        if (idx > 0)
+        if (need > 0)
 #pragma unroll
             for (int h = 0; h < NH; h++) fetch(0, h, pa[h]);
-        while (done < idx) {
+        while (done < need) {
             if (t == 0) {
                 unsigned long spins = 0;
                 int f;
@@ -911,7 +935,7 @@ __global__ __launch_bounds__(BS * 4) void backsub_kernel(const TF *__restrict__ 
                 s_front = f;
             }
             __syncthreads();
-            const int F = min(s_front, idx);
+            const int F = min(s_front, need);
             acquire_seq(strict);
 #pragma unroll
             for (int h = 0; h < NH; h++) accumulate(done, h, pa[h]);   // the prefetched tile
@@ -933,7 +957,7 @@ __global__ __launch_bounds__(BS * 4) void backsub_kernel(const TF *__restrict__ 
                 if (NH > 1) accumulate(d, NH - 1, v1);
             }
             done = F;
-            if (done < idx)
+            if (done < need)
 #pragma unroll
                 for (int h = 0; h < NH; h++) fetch(done, h, pa[h]);   // ahead of the next wait
             __syncthreads();
@@ -951,7 +975,33 @@ __global__ __launch_bounds__(BS * 4) void backsub_kernel(const TF *__restrict__ 
             // subtract U[row][rr] x[rr]; lane t holds rows t + 64 h
             TX xt[NH];
 #pragma unroll
-            for (int h = 0; h < NH; h++) xt[h] = (t + 64 * h < nrb) ? x[r0 + t + 64 * h] - part[t + 64 * h] : TX(0);
+            for (int h = 0; h < NH; h++) xt[h] = (t + 64 * h < nrb) ? yv[h] - part[t + 64 * h] : TX(0);
+            if constexpr (ADJ) {
+                if (idx > 0) {
+                    // the adjacent block: this wavefront alone waits for it
+                    unsigned long spins = 0;
+                    while ((int)ld_relaxed(front) < idx) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (((++spins) & 1023) == 0) {
+                            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                            if (spins > (1UL << 25)) {
+                                if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                break;
+                            }
+                        }
+                    }
+                    acquire_seq(strict);
+                    const int ca = r0 + BS + lane;
+                    const TX xa = (ca < n) ? ld_buf(xr, (int)(sizeof(TX) * ca), TX(0)) : TX(0);
+                    TX s0 = TX(0), s1 = TX(0);
+#pragma unroll
+                    for (int k = 0; k < BS; k += 2) {
+                        s0 += (TX)UaT[k][lane] * lane_bcast(xa, k);
+                        s1 += (TX)UaT[k + 1][lane] * lane_bcast(xa, k + 1);
+                    }
+                    xt[0] -= s0 + s1;
+                }
+            }
             for (int rr = nrb - 1; rr > 0; rr--) {
                 TX xk;
                 if (NH == 1 || rr < 64) xk = lane_bcast(xt[0], rr & 63);
