@@ -81,9 +81,10 @@ __global__ __launch_bounds__(256) void fwd_linv_kernel(double *r, int i, int nb,
     if (lane == 0) r[i + k] = s;
 }
 
-// r[j] -= sum_k A[j][i+k] * r[i+k] for rows j in [i+nb, n); one wavefront per row
-template <typename TF>
-__global__ __launch_bounds__(256) void l21_gemv_kernel(const TF *A, long lda, int n, int i, int nb, double *r)
+// r[j] -= sum_k A[j][i+k] * r[i+k] for rows j in [i+nb, n); one wavefront per
+// row, each lane reading four consecutive FP32 factors with one 16-byte load
+// (row segments 16-byte aligned, nb a multiple of 4: checked by the caller).
+__global__ __launch_bounds__(256) void l21_gemv_kernel(const float *A, long lda, int n, int i, int nb, double *r)
 {
     extern __shared__ double y[];
     for (int k = threadIdx.x; k < nb; k += 256) y[k] = r[i + k];
@@ -91,9 +92,15 @@ __global__ __launch_bounds__(256) void l21_gemv_kernel(const TF *A, long lda, in
     const int lane = threadIdx.x & 63;
     const long j = (long)i + nb + (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j >= n) return;
-    const TF *a = A + j * lda + i;
+    const float *a = A + j * lda + i;
     double s = 0.0;
-    for (int k = lane; k < nb; k += 64) s = fma((double)a[k], y[k], s);
+    for (int k = 4 * lane; k < nb; k += 256) {
+        const float4 v = *(const float4 *)(a + k);
+        s = fma((double)v.x, y[k], s);
+        s = fma((double)v.y, y[k + 1], s);
+        s = fma((double)v.z, y[k + 2], s);
+        s = fma((double)v.w, y[k + 3], s);
+    }
     s = wave_sum(s);
     if (lane == 0) r[j] -= s;
 }
@@ -223,6 +230,11 @@ struct Mixed {
     // r <- (LU)^-1 r with the FP32 factors, FP64 vector arithmetic
     int lu_solve(double *r)
     {
+        // the L21 products' 16-byte loads (n % nb == 0 and ldf = n + 16 give these)
+        if (ldf % 4 != 0 || nb % 4 != 0 || ((uintptr_t)F & 15) != 0) {
+            lu2mi_set_error_msg("mixed solve: FP32 factor rows not 16-byte aligned (ldf=%ld nb=%d)", ldf, nb);
+            return LU2MI_ERR_ARG;
+        }
         double *rb = scratch;   // nb values
         for (int i = 0; i < n; i += nb) {
             const int b = i / nb;
@@ -234,8 +246,8 @@ struct Mixed {
             LU2MI_CHECK(hipGetLastError());
             const int rows = n - i - nb;
             if (rows > 0) {
-                hipLaunchKernelGGL((l21_gemv_kernel<float>), dim3((rows + 3) / 4), dim3(256), sizeof(double) * nb,
-                                   s(), (const float *)F, ldf, n, i, nb, r);
+                hipLaunchKernelGGL(l21_gemv_kernel, dim3((rows + 3) / 4), dim3(256), sizeof(double) * nb, s(),
+                                   (const float *)F, ldf, n, i, nb, r);
                 LU2MI_CHECK(hipGetLastError());
             }
         }
