@@ -1,6 +1,6 @@
 """GPU parity: the HIP path against the CPU oracle and the golden fixtures.
 
-Bars (north star): pivot sequence identical to the reference LU; x within
+Bars: pivot sequence identical to the reference LU; x within
 1e-10 relative to ||x|| (gepp -- lu2's own COMPAT rule is numerically unstable,
 two correct implementations of it differ by up to ~1e-6 at n=2048, so COMPAT
 is held to identical pivots and 1e-5); HPL-2 scaled residual < 16 (gepp).
